@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark of the D3Q27 cumulant A-A lattice update on one MI355X.
+
+Headline workload (the value): FP64, 768^3 periodic box, seeded solenoidal
+turbulence (E(k) ~ k^4 exp(-2 (k/8)^2), u_rms = 0.04), tau = 0.5015, A-A
+pattern, default cumulant.  Per-precision lines (deviation storage FP32 /
+FP16 / BF16 and absolute FP32 at 512^3) and their rho/u error against an
+FP64 run of the same Taylor-Green vortex are reported beside it, as is the
+CPU oracle (a restatement of the reference's FP64 A-B host path) timed on a
+bounded sample on the host cores.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "tnl-lbm_amd"))
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "D3Q27 A-A MLUPS & %HBM roofline per precision; adjoint fwd+bwd MLUPS"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--size", type=int, default=768, help="edge of the FP64 headline box")
+    p.add_argument("--prec-size", type=int, default=512, help="edge of the per-precision boxes")
+    p.add_argument("--prec-steps", type=int, default=100)
+    p.add_argument("--err-size", type=int, default=128, help="edge of the TGV used for the error report")
+    p.add_argument("--err-steps", type=int, default=500)
+    p.add_argument("--no-precision", action="store_true")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    return p.parse_args()
+
+
+def timed_steps(lat, steps, warmup):
+    """W untimed steps, then K steps bracketed by synchronisation; per-launch
+    kernel durations from events recorded on the launch stream."""
+    import torch
+
+    for _ in range(warmup):
+        lat.step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record()
+        lat.step()
+        ev[i][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern = [a.elapsed_time(b) * 1e-3 for a, b in ev]
+    return wall, kern
+
+
+def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015):
+    import torch
+    from pytnl_lbm import D3Q27Lattice, _abi
+    from pytnl_lbm.fields import turbulence_fields_torch
+
+    lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0)
+    u = turbulence_fields_torch(N, seed, k0, u_rms, device="cuda")
+    rho = torch.ones_like(u[0])
+    lat.set_equilibrium_field(rho, *u)
+    del u, rho
+    torch.cuda.empty_cache()
+    wall, kern = timed_steps(lat, steps, warmup)
+    bytes_per_site = 2 * 27 * _abi.STORAGE_BYTES[lat.storage]
+    sites = N ** 3
+    avg_kernel = sum(kern) / len(kern)
+    res = {
+        "mlups": sites * steps / wall / 1e6,
+        "ms_per_step": wall / steps * 1e3,
+        "kernel_ms_avg": avg_kernel * 1e3,
+        "kernel_ms_even": sum(kern[0::2]) / max(1, len(kern[0::2])) * 1e3,
+        "kernel_ms_odd": sum(kern[1::2]) / max(1, len(kern[1::2])) * 1e3,
+        "bytes_per_site": bytes_per_site,
+        "achieved_GBs": bytes_per_site * sites / avg_kernel / 1e9,
+    }
+    res["roofline_frac"] = res["achieved_GBs"] / PEAK_HBM_GBS
+    # FP64 reductions of the final state (mass, NaN check)
+    lat.compute_macro()
+    red = lat.reduce_macro()
+    res["mass_rel_drift"] = abs(red[0] / sites - 1.0)
+    res["nan_sites"] = red[2]
+    del lat
+    torch.cuda.empty_cache()
+    return res
+
+
+def precision_errors(N, steps, storages):
+    """TGV (U0 = 0.05, tau = 0.505) run in FP64 and in each storage form;
+    L2 and max error of rho and u against the FP64 run after `steps` steps."""
+    import torch
+    from pytnl_lbm import D3Q27Lattice, tgv_fields
+
+    r = tgv_fields(N, N, N, 0.05)
+    nu = (0.505 - 0.5) / 3.0
+
+    def run(storage):
+        lat = D3Q27Lattice(N, N, N, storage=storage, nu=nu)
+        lat.set_equilibrium_field(*r)
+        for t in range(steps):
+            lat.step(output_macro=(t == steps - 1))
+        torch.cuda.synchronize()
+        out = [m.to(torch.float64).clone() for m in lat.macros()]
+        del lat
+        return out
+
+    ref = run("f64")
+    res = {}
+    for st in storages:
+        got = run(st)
+        e = {}
+        for name, a, b in [("rho", got[0], ref[0])]:
+            d = a - b
+            e["rho_L2"] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(b - 1.0))
+            e["rho_max"] = float(d.abs().max() / (b - 1.0).abs().max())
+        du = torch.stack([got[i] - ref[i] for i in (1, 2, 3)])
+        ub = torch.stack(ref[1:])
+        e["u_L2"] = float(torch.linalg.vector_norm(du) / torch.linalg.vector_norm(ub))
+        e["u_max"] = float(du.abs().max() / ub.abs().max())
+        res[st] = e
+    return res, {"tgv": f"{N}^3", "U0": 0.05, "tau": 0.505, "steps": steps,
+                 "rho_norm": "relative to rho-1 of the FP64 run"}
+
+
+def cpu_baseline(seconds):
+    """The oracle's FP64 A-B pull step (the reference default pattern,
+    defs.h:20-26), OpenMP collapse(2) over x,z as in state.hpp:1388-1397."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import numpy as np
+    import oracle as O
+    from pytnl_lbm import _abi
+    from pytnl_lbm.fields import turbulence_fields
+
+    threads = min(16, os.cpu_count() or 1)
+    N = 128
+    rho, vx, vy, vz = turbulence_fields(N, 1, 8.0 * N / 768, 0.04)
+    h = O.HostLattice(N, N, N, pattern=_abi.PATTERN_AB, nu=(0.5015 - 0.5) / 3.0, threads=threads)
+    h.set_equilibrium_field(rho, vx, vy, vz)
+    h.step()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        h.step()
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt > seconds and n >= 2:
+            break
+    return {"value": N ** 3 * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 D3Q27 cumulant A-B pull, {N}^3 periodic turbulence, {n} steps, "
+                      f"{threads} OpenMP threads"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is "
+              "described in DESIGN.md and not measured here", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    torch.cuda.init()
+    N = args.size
+    head = bench_case(N, "f64", args.steps, args.warmup)
+    out = {
+        "metric": METRIC,
+        "value": round(head["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded solenoidal turbulence, E(k)~k^4 exp(-2(k/8)^2), u_rms=0.04, rho=1",
+        "config": {"workload": f"D3Q27 cumulant A-A FP64, {N}^3 periodic box, tau=0.5015", "lattice": [N, N, N],
+                   "collision": "CUM", "pattern": "AA", "storage": "f64"},
+        "roofline": {"bound": "hbm", "achieved": round(head["achieved_GBs"], 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(head["roofline_frac"], 4), "traffic": None,
+                     "kernel": "k_aa_periodic<F64,CUM> (even/odd)", "bytes_per_site": head["bytes_per_site"],
+                     "kernel_ms_avg": round(head["kernel_ms_avg"], 3),
+                     "kernel_ms_even": round(head["kernel_ms_even"], 3),
+                     "kernel_ms_odd": round(head["kernel_ms_odd"], 3)},
+        "checks": {"mass_rel_drift": head["mass_rel_drift"], "nan_sites": head["nan_sites"]},
+    }
+    if not args.no_precision:
+        per = {}
+        P = args.prec_size
+        for st in ("f32", "f32dev", "f16dev", "bf16dev"):
+            r = bench_case(P, st, args.prec_steps, args.warmup)
+            per[f"{st}_{P}"] = {"mlups": round(r["mlups"], 1), "achieved_GBs": round(r["achieved_GBs"], 1),
+                                "roofline_frac": round(r["roofline_frac"], 4), "bytes_per_site": r["bytes_per_site"],
+                                "kernel_ms_avg": round(r["kernel_ms_avg"], 3)}
+        errs, spec = precision_errors(args.err_size, args.err_steps, ("f32", "f32dev", "f16dev", "bf16dev"))
+        for st, e in errs.items():
+            per[f"{st}_{P}"]["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in e.items()}
+        out["per_precision"] = per
+        out["precision_error_case"] = spec
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,281 @@
+/*
+ * tnl_lbm_amd.h -- C ABI of the MI355X (gfx950) lattice-update library.
+ *
+ * One shared library, libtnl_lbm_amd.so, replaces the reference's fused
+ * per-cell kernels (TNL-LBM include/lbm3d/kernels.h:59-196) and the device
+ * loops around them (include/lbm3d/lbm_block.hpp:248-305,
+ * include/lbm3d/state.hpp:1235-1409).  The reference instantiates one
+ * template per LBM_CONFIG<...> bundle (include/lbm3d/defs.h:288-310); a C ABI
+ * cannot carry templates, so every entry point here takes a small
+ * tnl_lbm_config that names the plugins at run time and the library
+ * dispatches to a kernel compiled for that combination.
+ *
+ * Conventions
+ *  - Plain C types only; all array pointers are DEVICE pointers owned by the
+ *    caller.  The library never allocates on a step path.
+ *  - Every function returns 0 (TNL_LBM_OK) or a negative status code and
+ *    never throws; tnl_lbm_last_error() returns a thread-local message.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Calls are asynchronous on that stream.
+ *  - Lattice layout is the reference's SoA layout (defs.h:90,
+ *    lbm_data.h:50-54): population q of site (x,y,z) lives at
+ *        q*XYZ + ((x+ox)*(Z+2oz) + (z+oz))*(Y+2oy) + (y+oy)
+ *    i.e. y is contiguous, then z, then x, then q.  With zero overlaps (one
+ *    GPU) this is q*XYZ + (x*Z + z)*Y + y.  The macro array uses the same
+ *    layout with the quantity id in place of q (lbm_data.h:56-60).
+ *  - Direction numbering is the reference enum (defs.h:317-350).
+ */
+#ifndef TNL_LBM_AMD_H
+#define TNL_LBM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TNL_LBM_ABI_VERSION 1u
+
+/* ---- status codes ------------------------------------------------------ */
+#define TNL_LBM_OK 0
+#define TNL_LBM_ERR_NULL (-1)        /* a required pointer is NULL            */
+#define TNL_LBM_ERR_DIMS (-2)        /* sizes / ranges are inconsistent        */
+#define TNL_LBM_ERR_UNSUPPORTED (-3) /* plugin combination not built          */
+#define TNL_LBM_ERR_VERSION (-4)     /* struct_size / version mismatch         */
+#define TNL_LBM_ERR_HIP (-5)         /* a HIP runtime call failed              */
+#define TNL_LBM_ERR_ARG (-6)         /* other invalid argument                 */
+
+/* ---- population directions (defs.h:317-350) --------------------------- */
+enum tnl_lbm_direction {
+	TNL_LBM_zzz = 0, TNL_LBM_pzz = 1, TNL_LBM_mzz = 2, TNL_LBM_zpz = 3, TNL_LBM_zmz = 4,
+	TNL_LBM_zzp = 5, TNL_LBM_zzm = 6, TNL_LBM_ppz = 7, TNL_LBM_mmz = 8, TNL_LBM_pmz = 9,
+	TNL_LBM_mpz = 10, TNL_LBM_pzp = 11, TNL_LBM_mzm = 12, TNL_LBM_pzm = 13, TNL_LBM_mzp = 14,
+	TNL_LBM_zpp = 15, TNL_LBM_zmm = 16, TNL_LBM_zpm = 17, TNL_LBM_zmp = 18, TNL_LBM_ppp = 19,
+	TNL_LBM_mmm = 20, TNL_LBM_ppm = 21, TNL_LBM_mmp = 22, TNL_LBM_pmp = 23, TNL_LBM_mpm = 24,
+	TNL_LBM_pmm = 25, TNL_LBM_mpp = 26
+};
+
+/* ---- D3Q27 geometry codes: D3Q27_BC_All::GEO (d3q27/bc.h:17-43) -------- */
+enum tnl_lbm_d3q27_geo {
+	TNL_LBM_GEO_FLUID = 0,
+	TNL_LBM_GEO_WALL = 1,
+	TNL_LBM_GEO_INFLOW = 2,
+	TNL_LBM_GEO_INFLOW_LEFT = 3,
+	TNL_LBM_GEO_INFLOW_BB_LEFT = 4,
+	TNL_LBM_GEO_INFLOW_EQ_LEFT = 5,
+	TNL_LBM_GEO_OUTFLOW_EQ = 6,
+	TNL_LBM_GEO_OUTFLOW_RIGHT = 7,
+	TNL_LBM_GEO_OUTFLOW_RIGHT_INTERP = 8,
+	TNL_LBM_GEO_PERIODIC = 9,
+	TNL_LBM_GEO_NOTHING = 10,
+	TNL_LBM_GEO_SYM_TOP = 11,
+	TNL_LBM_GEO_SYM_BOTTOM = 12,
+	TNL_LBM_GEO_SYM_LEFT = 13,
+	TNL_LBM_GEO_SYM_RIGHT = 14,
+	TNL_LBM_GEO_SYM_BACK = 15,
+	TNL_LBM_GEO_SYM_FRONT = 16
+};
+
+/* ---- D3Q7 geometry codes: D3Q7_BC_All::GEO (d3q7/bc.h:17-36) ----------- */
+enum tnl_lbm_d3q7_geo {
+	TNL_LBM_ADE_GEO_FLUID = 0,
+	TNL_LBM_ADE_GEO_WALL = 1,
+	TNL_LBM_ADE_GEO_WALL_BODY = 2,
+	TNL_LBM_ADE_GEO_SOLID = 3,
+	TNL_LBM_ADE_GEO_INFLOW = 7,
+	TNL_LBM_ADE_GEO_OUTFLOW_RIGHT = 8,
+	TNL_LBM_ADE_GEO_PERIODIC = 9,
+	TNL_LBM_ADE_GEO_NOTHING = 10,
+	TNL_LBM_ADE_GEO_SYM_TOP = 12,
+	TNL_LBM_ADE_GEO_SYM_BOTTOM = 13,
+	TNL_LBM_ADE_GEO_SYM_LEFT = 14,
+	TNL_LBM_ADE_GEO_SYM_RIGHT = 15,
+	TNL_LBM_ADE_GEO_SYM_BACK = 16,
+	TNL_LBM_ADE_GEO_SYM_FRONT = 17
+};
+
+/* ---- storage forms of the population array ----------------------------
+ * F64      : absolute f in double, FP64 arithmetic (TraitsDP, defs.h:177)
+ * F32      : absolute f in float,  FP32 arithmetic (TraitsSP, defs.h:176)
+ * F32_DEV  : deviation f - w_q stored in float, FP32 arithmetic
+ * F16_DEV  : deviation f - w_q stored in IEEE binary16, FP32 arithmetic
+ * BF16_DEV : deviation f - w_q stored in bfloat16, FP32 arithmetic
+ * The deviation forms follow the "well-conditioned" family of the reference
+ * (d3q27/common_well.h:37, d3q27/eq_well.h:21-134), which is not wired into
+ * any reference configuration; here it is a first-class storage form.      */
+enum tnl_lbm_storage {
+	TNL_LBM_STORE_F64 = 0,
+	TNL_LBM_STORE_F32 = 1,
+	TNL_LBM_STORE_F32_DEV = 2,
+	TNL_LBM_STORE_F16_DEV = 3,
+	TNL_LBM_STORE_BF16_DEV = 4
+};
+
+/* ---- macro / field element types -------------------------------------- */
+enum tnl_lbm_real { TNL_LBM_REAL_F64 = 0, TNL_LBM_REAL_F32 = 1 };
+
+/* ---- streaming patterns ------------------------------------------------
+ * AA : in-place A-A even/odd pattern (d3q27/streaming_AA.h:7-118), one copy
+ * AB : pull scheme with two copies   (d3q27/streaming_AB.h:7-60)           */
+enum tnl_lbm_pattern { TNL_LBM_PATTERN_AA = 0, TNL_LBM_PATTERN_AB = 1 };
+
+/* ---- D3Q27 collision operators (the COLL plugin) ----------------------- */
+enum tnl_lbm_collision {
+	TNL_LBM_COLL_CUM = 0,               /* D3Q27_CUM, default flags     col_cum.h:13-511        */
+	TNL_LBM_COLL_CUM_2017 = 1,          /* + USE_GEIER_CUM_2017          col_cum.h:112-329       */
+	TNL_LBM_COLL_CUM_2017_ANTIALIAS = 2,/* + USE_GEIER_CUM_ANTIALIAS     col_cum.h:234-245       */
+	TNL_LBM_COLL_MRT_LES = 3,           /* D3Q27_MRT ("MRT_LES")         col_mrt.h:7-142         */
+	TNL_LBM_COLL_KBC_N1 = 4,            /* D3Q27_KBC_N1..N4              col_kbc_n.h:254-1272    */
+	TNL_LBM_COLL_KBC_N2 = 5,
+	TNL_LBM_COLL_KBC_N3 = 6,
+	TNL_LBM_COLL_KBC_N4 = 7,
+	TNL_LBM_COLL_KBC_C1 = 8,            /* D3Q27_KBC_C1..C4              col_kbc_c.h:283-1301    */
+	TNL_LBM_COLL_KBC_C2 = 9,
+	TNL_LBM_COLL_KBC_C3 = 10,
+	TNL_LBM_COLL_KBC_C4 = 11
+};
+
+/* ---- D3Q7 collision operators ------------------------------------------ */
+enum tnl_lbm_ade_collision {
+	TNL_LBM_ADE_COLL_SRT = 0,  /* D3Q7_SRT  d3q7/col_srt.h:8-27   */
+	TNL_LBM_ADE_COLL_MRT = 1,  /* D3Q7_MRT  d3q7/col_mrt.h:8-51   */
+	TNL_LBM_ADE_COLL_CLBM = 2  /* D3Q7_CLBM d3q7/col_clbm.h:10-85 */
+};
+
+/* ---- step flags --------------------------------------------------------- */
+#define TNL_LBM_FLAG_OUTPUT_MACRO 0x1u  /* output_macro argument (kernels.h:66,110) */
+#define TNL_LBM_FLAG_DISTRIBUTED_X 0x2u /* bool3d distributed (kernels.h:9,22-27)  */
+#define TNL_LBM_FLAG_DISTRIBUTED_Y 0x4u
+#define TNL_LBM_FLAG_DISTRIBUTED_Z 0x8u
+
+/* Plugin bundle: the run-time image of LBM_CONFIG<TRAITS, KernelStruct,
+ * DATA, COLL, EQ, STREAMING, BC, MACRO> (defs.h:288-310).  EQ is the one
+ * the collision implies (D3Q27_EQ_INV_CUM for the cumulants, D3Q27_EQ for
+ * MRT, the entropic equilibrium for KBC); BC is D3Q27_BC_All; MACRO is
+ * D3Q27_MACRO_Default (rho, vx, vy, vz).                                   */
+typedef struct tnl_lbm_config {
+	uint32_t struct_size; /* sizeof(tnl_lbm_config) */
+	uint32_t version;     /* TNL_LBM_ABI_VERSION    */
+	int32_t collision;    /* enum tnl_lbm_collision */
+	int32_t storage;      /* enum tnl_lbm_storage   */
+	int32_t pattern;      /* enum tnl_lbm_pattern   */
+	int32_t macro_real;   /* enum tnl_lbm_real: element type of `macro` */
+} tnl_lbm_config;
+
+/* Kernel data: the run-time image of NSE_Data_ConstInflow /
+ * NSE_Data_InflowProfile (lbm_data.h:8-133), passed by pointer.            */
+typedef struct tnl_lbm_d3q27_data {
+	uint32_t struct_size; /* sizeof(tnl_lbm_d3q27_data) */
+	uint32_t version;     /* TNL_LBM_ABI_VERSION        */
+	int64_t X, Y, Z;      /* local sizes without overlaps (LBM_Data::X/Y/Z) */
+	int64_t ox, oy, oz;   /* overlaps (indexer overlaps; 0 on one GPU)      */
+	int64_t XYZ;          /* population stride; >= (X+2ox)(Y+2oy)(Z+2oz)    */
+	int32_t even_iter;    /* A-A parity, LBM::updateKernelData (lbm.hpp:311) */
+	int32_t reserved0;
+	double lbm_viscosity; /* LBM_Data::lbmViscosity                          */
+	double fx, fy, fz;    /* NSE_Data homogeneous force (lbm_data.h:69-72)   */
+	void* df[2];          /* dfs[df_cur], dfs[df_out] (df_out: A-B only)     */
+	void* macro;          /* dmacro: 4*XYZ elements of macro_real, or NULL   */
+	const int16_t* map;   /* dmap: XYZ geometry codes; NULL = all PERIODIC   */
+	double inflow_vx, inflow_vy, inflow_vz; /* NSE_Data_ConstInflow (lbm_data.h:81-91) */
+	/* NSE_Data_InflowProfile (lbm_data.h:100-132): double arrays indexed
+	 * Y*z + y; a non-NULL pointer overrides the constant component.        */
+	const double* inflow_profile[3];
+} tnl_lbm_d3q27_data;
+
+/* D3Q7 ADE data: ADE_Data (lbm_data.h:212-246) plus a source term that the
+ * reference leaves as a TODO (lbm_data.h:229).                             */
+typedef struct tnl_lbm_d3q7_data {
+	uint32_t struct_size; /* sizeof(tnl_lbm_d3q7_data) */
+	uint32_t version;
+	int64_t X, Y, Z;
+	int64_t ox, oy, oz;
+	int64_t XYZ;
+	int32_t even_iter;
+	int32_t collision;       /* enum tnl_lbm_ade_collision                    */
+	double lbm_diffusivity;  /* ADE_Data::lbmViscosity (the diffusivity D)    */
+	void* df[2];             /* 7*XYZ elements of the NSE storage form        */
+	void* macro;             /* phi: XYZ elements of macro_real, or NULL      */
+	const int16_t* map;      /* D3Q7 geometry codes; NULL = all PERIODIC       */
+	const void* diffusion_coefficient; /* per-site D (macro_real), or NULL    */
+	const void* source;      /* per-site source S (macro_real), or NULL; adds
+	                            w_q*S to every population after collision     */
+	double inflow_phi;       /* ADE_Data_ConstInflow::inflow_phi               */
+	/* deviation storage forms only: stored = (f_q - w_q*phi_ref)*store_scale */
+	double phi_ref;
+	double store_scale;
+} tnl_lbm_d3q7_data;
+
+/* ======================================================================= */
+/* library information                                                     */
+uint32_t tnl_lbm_abi_version(void);
+const char* tnl_lbm_last_error(void);
+
+/* ---- D3Q27 lattice update ----------------------------------------------
+ * Replaces one launch of cudaLBMKernel<NSE>(SD, offset, end, distributed,
+ * output_macro) (kernels.h:59-112, launched at state.hpp:1307-1309) over the
+ * box offset <= (x,y,z) < end.  flags: TNL_LBM_FLAG_*.                     */
+int tnl_lbm_d3q27_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
+                       const int64_t offset[3], const int64_t end[3], uint32_t flags, void* stream);
+
+/* ---- coupled NSE + ADE update ------------------------------------------
+ * Replaces cudaLBMKernel<NSE,ADE> (kernels.h:114-196): the D3Q27 step, then
+ * the D3Q7 step at the same site driven by the NSE velocity.               */
+int tnl_lbm_nse_ade_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* nse,
+                         const tnl_lbm_d3q7_data* ade, const int64_t offset[3], const int64_t end[3],
+                         uint32_t flags, void* stream);
+
+/* ---- initialisation -----------------------------------------------------
+ * set_equilibrium: LBM_BLOCK::setEquilibrium(rho, vx, vy, vz)
+ *   (lbm_block.hpp:248-278): f = f_eq(rho,u) at every stored site, including
+ *   overlaps; with A-B both copies are written.
+ * set_equilibrium_field: COLL::setEquilibriumLat over a field
+ *   (common.h:126-158, used by the initial conditions, sim_4.cu:145-163):
+ *   rho/vx/vy/vz are DEVICE double arrays of XYZ elements in the lattice
+ *   layout.
+ * compute_macro: LBM_BLOCK::computeInitialMacro (lbm_block.hpp:281-305).  */
+int tnl_lbm_d3q27_set_equilibrium(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
+                                  double rho, double vx, double vy, double vz, void* stream);
+int tnl_lbm_d3q27_set_equilibrium_field(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
+                                        const double* rho, const double* vx, const double* vy,
+                                        const double* vz, void* stream);
+int tnl_lbm_d3q27_compute_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, void* stream);
+
+/* ---- population transfer ------------------------------------------------
+ * Convert between the stored form (storage dtype, deviation offset) and
+ * absolute double populations in the same q*XYZ layout, slot by slot (no
+ * A-A reordering).  `out`/`in` are device arrays of 27*XYZ doubles;
+ * `which` selects df[0] or df[1].                                          */
+int tnl_lbm_d3q27_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int which,
+                             double* out, void* stream);
+int tnl_lbm_d3q27_import_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int which,
+                             const double* in, void* stream);
+
+/* ---- D3Q7 initialisation (ADE) ------------------------------------------
+ * f_q = D3Q7_EQ::eq_q(phi, 0) (d3q7/common.h:105-115) at every stored site:
+ * phi is a device array of XYZ elements (macro_real) or NULL for phi0.
+ * export_f64 returns absolute populations (7*XYZ doubles).                 */
+int tnl_lbm_d3q7_set_equilibrium(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* ADE,
+                                 const void* phi, double phi0, void* stream);
+int tnl_lbm_d3q7_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* ADE, int which,
+                            double* out, void* stream);
+
+/* ---- FP64 global reductions ---------------------------------------------
+ * Sums over the local box of macro quantities, accumulated in double
+ * regardless of the storage form (the reference's `real` is double,
+ * defs.h:77; sim_4.cu:257-300, state.hpp:1469-1492).  out[0..5] receives:
+ *   [0] sum rho              (mass)
+ *   [1] sum rho*|u|^2        (twice the kinetic energy)
+ *   [2] count of sites with rho != rho (NaN check)
+ *   [3] sum (rho-1)^2
+ *   [4] sum |u|^2
+ *   [5] number of sites summed
+ * `out` is a HOST pointer; the call synchronises `stream`.                 */
+int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
+                               double out[6], void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNL_LBM_AMD_H */

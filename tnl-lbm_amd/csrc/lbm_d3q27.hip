@@ -1,0 +1,1180 @@
+// gfx950 kernels of the D3Q27 lattice update and the C ABI that drives them.
+//
+// Replaces cudaLBMKernel<NSE> (TNL-LBM include/lbm3d/kernels.h:59-112) and
+// the device loops of LBM_BLOCK (lbm_block.hpp:248-305).  Design:
+//  * one lattice site per lane, 64-lane waves laid along y (the contiguous
+//    axis), so each of the 27 loads and 27 stores of a wave is one
+//    contiguous 64-element segment of one population plane;
+//  * a block spans one (x, z) row segment, so the row offsets of the site
+//    and of its x/z neighbours are wave-uniform and live in scalar
+//    registers; only the y index (and its +-1 neighbours) is per lane;
+//  * the 1-D grid is remapped so that consecutive row segments land on the
+//    same XCD (workgroups are dispatched round-robin over the 8 XCDs), which
+//    keeps the cache lines shared by y-neighbouring waves in one L2;
+//  * the pure-periodic A-A path reads no geometry map at all; the general
+//    path reads the int16 map and applies D3Q27_BC_All (d3q27/bc.h:61-422).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "lbm_ade.hpp"
+#include "lbm_device.hpp"
+
+using namespace lbm;
+
+namespace {
+
+thread_local std::string g_err;
+int fail(int code, const std::string& msg)
+{
+	g_err = msg;
+	return code;
+}
+
+// ---------------------------------------------------------------------------
+struct StepArgs
+{
+	void* df0;
+	void* df1;
+	void* macro;
+	const int16_t* map;
+	int64_t XYZ;
+	int64_t X, Y, Z;     // sizes without overlaps
+	int64_t ox, oy, oz;  // overlaps
+	int64_t Ys, Zs;      // stored extents Y+2oy, Z+2oz
+	int32_t offx, offy, offz;
+	int32_t nx, ny, nz;
+	uint32_t nyb;      // y segments per row
+	uint32_t nblocks;  // logical blocks
+	uint32_t per_xcd;  // padded blocks / 8
+	int32_t even;
+	int32_t out_macro;
+	int32_t macro_f64;
+	int32_t dist_x, dist_y, dist_z;
+	double omega1;
+	double fx, fy, fz;
+	double inflow[3];
+	const double* prof[3];
+};
+
+constexpr int NXCD = 8;
+
+// logical block -> (x, z, y segment) with XCD-aware remapping
+LBM_DEV bool block_coords(const StepArgs& A, int64_t& x, int64_t& z, int64_t& y)
+{
+	const uint32_t b = blockIdx.x;
+	const uint32_t L = (b % NXCD) * A.per_xcd + b / NXCD;
+	if (L >= A.nblocks)
+		return false;
+	const uint32_t yb = L % A.nyb;
+	const uint32_t t = L / A.nyb;
+	z = A.offz + (int64_t) (t % (uint32_t) A.nz);
+	x = A.offx + (int64_t) (t / (uint32_t) A.nz);
+	y = A.offy + (int64_t) yb * blockDim.x + threadIdx.x;
+	return y < A.offy + A.ny;
+}
+
+template <typename E>
+LBM_DEV E* at(E* base, uint32_t i)
+{
+	// uniform 64-bit base + 32-bit per-lane byte offset (global saddr form)
+	return (E*) ((char*) base + (uint32_t) (i * (uint32_t) sizeof(E)));
+}
+
+template <typename real>
+LBM_DEV void write_macro(const StepArgs& A, int64_t s, real rho, real vx, real vy, real vz)
+{
+	if (A.macro_f64) {
+		double* m = (double*) A.macro;
+		m[s] = rho;
+		m[A.XYZ + s] = vx;
+		m[2 * A.XYZ + s] = vy;
+		m[3 * A.XYZ + s] = vz;
+	}
+	else {
+		float* m = (float*) A.macro;
+		m[s] = (float) rho;
+		m[A.XYZ + s] = (float) vx;
+		m[2 * A.XYZ + s] = (float) vy;
+		m[3 * A.XYZ + s] = (float) vz;
+	}
+}
+
+template <int ST, int COLL>
+LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real rho, typename Storage<ST>::real vx,
+					 typename Storage<ST>::real vy, typename Storage<ST>::real vz, typename Storage<ST>::real omega1)
+{
+	using P = Phys<ST>;
+	if constexpr (Storage<ST>::dev)
+		P::cumulant(f, rho, vx, vy, vz, omega1);
+	else if constexpr (COLL == TNL_LBM_COLL_CUM)
+		P::template cumulant<false, false>(f, rho, vx, vy, vz, omega1);
+	else if constexpr (COLL == TNL_LBM_COLL_CUM_2017)
+		P::template cumulant<true, false>(f, rho, vx, vy, vz, omega1);
+	else
+		P::template cumulant<true, true>(f, rho, vx, vy, vz, omega1);
+}
+
+// ===========================================================================
+// Fast path: A-A pattern, every site GEO_PERIODIC (no map).
+// Even step (streaming_AA.h:81-84, 16-45): read own slots, write opposite.
+// Odd step (streaming_AA.h:86-115, 46-75): pull from r - c_q (opposite
+// slot), push to r + c_q (same slot).
+// ===========================================================================
+template <int ST, int COLL, bool EVEN>
+__global__ void __launch_bounds__(256) k_aa_periodic(const StepArgs A)
+{
+	using S = Storage<ST>;
+	using real = typename S::real;
+	using elem = typename S::elem;
+	int64_t x, z, y;
+	if (! block_coords(A, x, z, y))
+		return;
+	elem* __restrict__ df = (elem*) A.df0;
+	real f[27];
+	const uint32_t yl = (uint32_t) (y + A.oy);
+	if constexpr (EVEN) {
+		const int64_t row = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys;
+#pragma unroll
+		for (int q = 0; q < 27; q++)
+			f[q] = S::ld(at(df + (q * A.XYZ + row), yl));
+		real rho, vx, vy, vz;
+		Phys<ST>::density_velocity(f, (real) A.fx, (real) A.fy, (real) A.fz, rho, vx, vy, vz);
+		collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
+#pragma unroll
+		for (int q = 0; q < 27; q++)
+			S::st(at(df + (opp(q) * A.XYZ + row), yl), f[q]);
+		if (A.out_macro)
+			write_macro(A, row + yl, rho, vx, vy, vz);
+	}
+	else {
+		// periodic neighbours (kernelInitIndices, kernels.h:21-28)
+		const int64_t xp = (! A.dist_x && x == A.X - 1) ? 0 : x + 1;
+		const int64_t xm = (! A.dist_x && x == 0) ? A.X - 1 : x - 1;
+		const int64_t zp = (! A.dist_z && z == A.Z - 1) ? 0 : z + 1;
+		const int64_t zm = (! A.dist_z && z == 0) ? A.Z - 1 : z - 1;
+		const uint32_t ypl = (uint32_t) (((! A.dist_y && y == A.Y - 1) ? 0 : y + 1) + A.oy);
+		const uint32_t yml = (uint32_t) (((! A.dist_y && y == 0) ? A.Y - 1 : y - 1) + A.oy);
+		const int64_t xs[3] = {xm + A.ox, x + A.ox, xp + A.ox};
+		const int64_t zs[3] = {zm + A.oz, z + A.oz, zp + A.oz};
+		const uint32_t ys[3] = {yml, yl, ypl};
+#pragma unroll
+		for (int q = 0; q < 27; q++) {
+			const int64_t row = (xs[1 - CX[q]] * A.Zs + zs[1 - CZ[q]]) * A.Ys;
+			f[q] = S::ld(at(df + (opp(q) * A.XYZ + row), ys[1 - CY[q]]));
+		}
+		real rho, vx, vy, vz;
+		Phys<ST>::density_velocity(f, (real) A.fx, (real) A.fy, (real) A.fz, rho, vx, vy, vz);
+		collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
+#pragma unroll
+		for (int q = 0; q < 27; q++) {
+			const int64_t row = (xs[1 + CX[q]] * A.Zs + zs[1 + CZ[q]]) * A.Ys;
+			S::st(at(df + (q * A.XYZ + row), ys[1 + CY[q]]), f[q]);
+		}
+		if (A.out_macro)
+			write_macro(A, (xs[1] * A.Zs + zs[1]) * A.Ys + yl, rho, vx, vy, vz);
+	}
+}
+
+// ===========================================================================
+// General path: geometry map + D3Q27_BC_All, A-A or A-B.
+// ===========================================================================
+enum : int16_t
+{
+	GEO_FLUID = 0, GEO_WALL, GEO_INFLOW, GEO_INFLOW_LEFT, GEO_INFLOW_BB_LEFT, GEO_INFLOW_EQ_LEFT, GEO_OUTFLOW_EQ,
+	GEO_OUTFLOW_RIGHT, GEO_OUTFLOW_RIGHT_INTERP, GEO_PERIODIC, GEO_NOTHING, GEO_SYM_TOP, GEO_SYM_BOTTOM, GEO_SYM_LEFT,
+	GEO_SYM_RIGHT, GEO_SYM_BACK, GEO_SYM_FRONT
+};
+
+struct Nb
+{
+	int64_t m, o, p;  // storage coordinates (overlap-shifted) of c = -1, 0, +1
+	LBM_DEV int64_t operator[](int c) const { return c < 0 ? m : (c > 0 ? p : o); }
+};
+
+// kernelInitIndices (kernels.h:5-57).  A-A indices of non-periodic sites
+// that would leave the stored array wrap instead (the reference reads
+// outside its allocation there).
+template <bool AA>
+LBM_DEV Nb neighbours(int64_t c, int64_t n, int64_t o, bool dist, bool periodic)
+{
+	int64_t p, m;
+	if (periodic) {
+		p = (! dist && c == n - 1) ? 0 : c + 1;
+		m = (! dist && c == 0) ? n - 1 : c - 1;
+	}
+	else if (AA) {
+		p = c + 1 < n + o ? c + 1 : 0;
+		m = c - 1 >= -o ? c - 1 : n - 1;
+	}
+	else {
+		p = c + 1 < n - 1 + o ? c + 1 : n - 1 + o;
+		m = c - 1 > -o ? c - 1 : -o;
+	}
+	return Nb{m + o, c + o, p + o};
+}
+
+template <int ST, int COLL, int PATTERN>
+struct General
+{
+	using S = Storage<ST>;
+	using real = typename S::real;
+	using elem = typename S::elem;
+	using P = Phys<ST>;
+	static constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
+	static constexpr int EQK = (COLL <= TNL_LBM_COLL_CUM_2017_ANTIALIAS) ? 0 : 1;
+
+	static LBM_DEV int64_t sidx(const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
+	{
+		return (xs * A.Zs + zs) * A.Ys + ys;
+	}
+
+	static LBM_DEV void set_eq(real* f, real rho, real vx, real vy, real vz)
+	{
+#pragma unroll
+		for (int q = 0; q < 27; q++)
+			f[q] = eq_q(q, rho, vx, vy, vz);
+	}
+	static LBM_DEV real eq_q(int q, real rho, real vx, real vy, real vz)
+	{
+		// q is a compile-time constant after unrolling
+		switch (q) {
+#define EQC(Q) case Q: return P::template eq<EQK, Q>(rho, vx, vy, vz);
+			EQC(0) EQC(1) EQC(2) EQC(3) EQC(4) EQC(5) EQC(6) EQC(7) EQC(8) EQC(9) EQC(10) EQC(11) EQC(12) EQC(13)
+			EQC(14) EQC(15) EQC(16) EQC(17) EQC(18) EQC(19) EQC(20) EQC(21) EQC(22) EQC(23) EQC(24) EQC(25) EQC(26)
+#undef EQC
+		}
+		return 0;
+	}
+
+	static LBM_DEV void inflow(const StepArgs& A, int64_t y, int64_t z, real& vx, real& vy, real& vz)
+	{
+		// NSE_Data_ConstInflow / NSE_Data_InflowProfile::inflow (lbm_data.h:85-132)
+		const int64_t i = A.Y * z + y;
+		vx = (real) (A.prof[0] ? A.prof[0][i] : A.inflow[0]);
+		vy = (real) (A.prof[1] ? A.prof[1][i] : A.inflow[1]);
+		vz = (real) (A.prof[2] ? A.prof[2][i] : A.inflow[2]);
+	}
+
+	static LBM_DEV void site(const StepArgs& A, int64_t x, int64_t y, int64_t z)
+	{
+		real vx, vy, vz;
+		Nb NX, NY, NZ;
+		site(A, x, y, z, vx, vy, vz, NX, NY, NZ);
+	}
+
+	// one site; returns the pre-collision velocity and the neighbour indices
+	// (the coupled kernel drives the ADE step with them, kernels.h:152-176)
+	static LBM_DEV void site(const StepArgs& A, int64_t x, int64_t y, int64_t z, real& ovx, real& ovy, real& ovz, Nb& NX,
+							 Nb& NY, Nb& NZ)
+	{
+		const int64_t s0 = sidx(A, x + A.ox, y + A.oy, z + A.oz);
+		const int16_t gi = A.map ? A.map[s0] : (int16_t) GEO_PERIODIC;
+		const bool per = gi == GEO_PERIODIC;
+		NX = neighbours<AA>(x, A.X, A.ox, A.dist_x, per);
+		NY = neighbours<AA>(y, A.Y, A.oy, A.dist_y, per);
+		NZ = neighbours<AA>(z, A.Z, A.oz, A.dist_z, per);
+		const real Fx = (real) A.fx, Fy = (real) A.fy, Fz = (real) A.fz;
+		real f[27];
+		real rho = 1, vx = 0, vy = 0, vz = 0;
+		const elem* __restrict__ cur = (const elem*) A.df0;
+
+		if (gi != GEO_NOTHING) {
+			// pull location (bc.h:72-74); OUTFLOW_RIGHT pulls as if located at x-1
+			Nb PX = NX;
+			if (gi == GEO_OUTFLOW_RIGHT)
+				PX.p = PX.o = NX.m;
+			if (gi == GEO_OUTFLOW_RIGHT_INTERP) {
+				// streamingInterpRight (streaming_AB.h:446-478)
+				const real cs = (real) 0.5773502691896257;
+#pragma unroll
+				for (int q = 0; q < 27; q++) {
+					const int64_t yy = NY[-CY[q]], zz = NZ[-CZ[q]];
+					if (CX[q] < 0)
+						f[q] = cs * S::ld(cur + q * A.XYZ + sidx(A, NX.m, yy, zz)) + ((real) 1 - cs) * S::ld(cur + q * A.XYZ + sidx(A, NX.o, yy, zz));
+					else
+						f[q] = S::ld(cur + q * A.XYZ + sidx(A, CX[q] > 0 ? NX.m : NX.o, yy, zz));
+				}
+			}
+			else if (AA && A.even) {
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX.o, NY.o, NZ.o));
+			}
+			else if (AA) {
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					f[q] = S::ld(cur + opp(q) * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+			}
+			else {
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+			}
+
+			const real rest = S::dev ? (real) 1 : (real) 0;
+			switch (gi) {
+				case GEO_INFLOW:
+					inflow(A, y, z, vx, vy, vz);
+					rho = 1;
+					set_eq(f, rho, vx, vy, vz);
+					break;
+				case GEO_INFLOW_LEFT:
+					if constexpr (! S::dev) {
+						// moment-based inflow (bc.h:91-142)
+						inflow(A, y, z, vx, vy, vz);
+						rho = (real) 1 / (1 - vx)
+							* ((f[zzz] + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[zpz] + f[zmz]) + (f[zzp] + f[zzm]))))
+							   + 2 * (f[mzz] + (((f[mpp] + f[mmm]) + (f[mpm] + f[mmp])) + ((f[mpz] + f[mmz]) + (f[mzp] + f[mzm])))));
+						const real n1o3 = (real) (1.0 / 3.0), n1o9 = (real) (1.0 / 9.0);
+						const real m100 = rho * vx, m010 = rho * vy, m001 = rho * vz, m011 = rho * (vy * vz);
+						const real m020 = n1o3 * rho + rho * (vy * vy);
+						const real m002 = n1o3 * rho + rho * (vz * vz);
+						const real m021 = n1o3 * rho * vz + rho * ((vy * vy) * vz);
+						const real m012 = n1o3 * rho * vy + rho * (vy * (vz * vz));
+						const real m022 = n1o9 * rho + n1o3 * rho * (vy * vy + vz * vz) + rho * (vy * vy) * (vz * vz);
+						f[pzz] = m100 + (m022 - (m020 + m002)) + f[mzz]
+							   + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[zzp] + f[zzm]) + (f[zpz] + f[zmz])))
+							   + 2 * (((f[mpp] + f[mmm]) + (f[mpm] + f[mmp])) + ((f[mpz] + f[mmz]) + (f[mzp] + f[mzm])));
+						f[ppz] = (real) 0.5 * ((m020 - m022) + (-m012 + m010)) - (f[mpz] + f[zpz]);
+						f[pmz] = (real) 0.5 * ((m020 - m022) + (m012 - m010)) - (f[mmz] + f[zmz]);
+						f[pzp] = (real) 0.5 * ((m002 - m022) + (-m021 + m001)) - (f[mzp] + f[zzp]);
+						f[pzm] = (real) 0.5 * ((m002 - m022) + (m021 - m001)) - (f[mzm] + f[zzm]);
+						f[ppp] = (real) 0.25 * ((m022 + m011) + (m021 + m012)) - (f[mpp] + f[zpp]);
+						f[ppm] = (real) 0.25 * ((m022 - m011) + (-m021 + m012)) - (f[mpm] + f[zpm]);
+						f[pmp] = (real) 0.25 * ((m022 - m011) + (m021 - m012)) - (f[mmp] + f[zmp]);
+						f[pmm] = (real) 0.25 * ((m022 + m011) + (-m021 - m012)) - (f[mmm] + f[zmm]);
+					}
+					else
+						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+					break;
+				case GEO_INFLOW_BB_LEFT:
+					{
+						// bc.h:143-165 (pull scheme): reads unstreamed own slots
+						inflow(A, y, z, vx, vy, vz);
+						const real k2 = (real) 2 * (real) 3;
+						const int64_t so = sidx(A, NX.o, NY.o, NZ.o);
+						f[pzz] = S::ld(cur + mzz * A.XYZ + so) - k2 * (real) (2.0 / 27.0) * rho * (-vx);
+						f[ppz] = S::ld(cur + mmz * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vy);
+						f[pmz] = S::ld(cur + mpz * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vy);
+						f[pzp] = S::ld(cur + mzm * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vz);
+						f[pzm] = S::ld(cur + mzp * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vz);
+						f[ppp] = S::ld(cur + mmm * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy - vz);
+						f[ppm] = S::ld(cur + mmp * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy + vz);
+						f[pmp] = S::ld(cur + mpm * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy - vz);
+						f[pmm] = S::ld(cur + mpp * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy + vz);
+						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+						break;
+					}
+				case GEO_INFLOW_EQ_LEFT:
+					// bc.h:166-185 (deviation storage: the known-direction weights sum to 1)
+					inflow(A, y, z, vx, vy, vz);
+					rho = (real) 1 / (1 - vx)
+						* ((rest + (f[zzz] + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[zpz] + f[zmz]) + (f[zzp] + f[zzm])))))
+						   + 2 * (f[mzz] + (((f[mpp] + f[mmm]) + (f[mpm] + f[mmp])) + ((f[mpz] + f[mmz]) + (f[mzp] + f[mzm])))));
+					set_eq(f, rho, vx, vy, vz);
+					break;
+				case GEO_OUTFLOW_EQ:
+					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+					rho = 1;
+					set_eq(f, rho, vx, vy, vz);
+					break;
+				case GEO_OUTFLOW_RIGHT:
+					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+					rho = 1;
+					break;
+				case GEO_OUTFLOW_RIGHT_INTERP:
+					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+#pragma unroll
+					for (int q = 0; q < 27; q++)
+						f[q] += eq_q(q, (real) 1, vx, vy, vz) - eq_q(q, rho, vx, vy, vz);
+					rho = 1;
+					break;
+				case GEO_WALL:
+					// full-way bounce-back (bc.h:201-221)
+					rho = 1;
+					vx = vy = vz = 0;
+#pragma unroll
+					for (int q = 1; q < 27; q += 2) {
+						const real t = f[q];
+						f[q] = f[q + 1];
+						f[q + 1] = t;
+					}
+					break;
+				case GEO_SYM_TOP:
+				case GEO_SYM_BOTTOM:
+				case GEO_SYM_LEFT:
+				case GEO_SYM_RIGHT:
+				case GEO_SYM_BACK:
+				case GEO_SYM_FRONT:
+					{
+						// bc.h:222-293
+						const int axis = (gi == GEO_SYM_TOP || gi == GEO_SYM_BOTTOM) ? 2 : (gi == GEO_SYM_LEFT || gi == GEO_SYM_RIGHT) ? 0 : 1;
+						const int sgn = (gi == GEO_SYM_BOTTOM || gi == GEO_SYM_LEFT || gi == GEO_SYM_BACK) ? 1 : -1;
+#pragma unroll
+						for (int q = 0; q < 27; q++) {
+							const int c = axis == 0 ? CX[q] : axis == 1 ? CY[q] : CZ[q];
+							if (c == sgn) {
+								const int mq = axis == 0 ? qidx(1 - sgn, CY[q] + 1, CZ[q] + 1)
+											 : axis == 1 ? qidx(CX[q] + 1, 1 - sgn, CZ[q] + 1)
+														 : qidx(CX[q] + 1, CY[q] + 1, 1 - sgn);
+								real src = 0;
+#pragma unroll
+								for (int r = 0; r < 27; r++)
+									if (r == mq)
+										src = f[r];
+								f[q] = src;
+							}
+						}
+						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+						break;
+					}
+				default:
+					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+					break;
+			}
+			// doCollision (bc.h:406-412)
+			const bool do_coll = gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_RIGHT_INTERP
+							  || gi == GEO_INFLOW_LEFT || gi == GEO_INFLOW_BB_LEFT;
+			if (do_coll)
+				collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
+			// postCollision streaming
+			if (AA && A.even) {
+				elem* __restrict__ out = (elem*) A.df0;
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					S::st(out + opp(q) * A.XYZ + s0, f[q]);
+			}
+			else if (AA) {
+				elem* __restrict__ out = (elem*) A.df0;
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					S::st(out + q * A.XYZ + sidx(A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q]);
+			}
+			else {
+				elem* __restrict__ out = (elem*) A.df1;
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					S::st(out + q * A.XYZ + s0, f[q]);
+			}
+		}
+		if (A.out_macro)
+			write_macro(A, s0, rho, vx, vy, vz);
+		ovx = vx;
+		ovy = vy;
+		ovz = vz;
+	}
+};
+
+template <int ST, int COLL, int PATTERN>
+__global__ void __launch_bounds__(256) k_general(const StepArgs A)
+{
+	int64_t x, z, y;
+	if (! block_coords(A, x, z, y))
+		return;
+	General<ST, COLL, PATTERN>::site(A, x, y, z);
+}
+
+// ===========================================================================
+// Coupled NSE + ADE: cudaLBMKernel<NSE,ADE> (kernels.h:114-196)
+// ===========================================================================
+template <int ST, int PATTERN>
+LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename Storage<ST>::real vx,
+					  typename Storage<ST>::real vy, typename Storage<ST>::real vz, const Nb& NX, const Nb& NY, const Nb& NZ)
+{
+	using K = Ade7<ST>;
+	using real = typename K::real;
+	using elem = typename K::elem;
+	constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
+	const int16_t gi = B.map ? B.map[s0] : (int16_t) A_PERIODIC;
+	const real pr = (real) B.phi_ref, sc = (real) B.scale;
+	auto macro_at = [&](const void* p) -> real {
+		return A.macro_f64 ? (real) ((const double*) p)[s0] : (real) ((const float*) p)[s0];
+	};
+	// D3Q7_MACRO_Default::copyQuantities (d3q7/macro.h:38-42)
+	const real D = B.diff ? macro_at(B.diff) : (real) B.D;
+	real phi = 0;
+	if (gi != A_NOTHING) {
+		const elem* cur = (const elem*) B.df0;
+		auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
+		Nb PX = NX;
+		if (gi == A_OUTFLOW_RIGHT)
+			PX.p = PX.o = NX.m;
+		real f[7];
+#pragma unroll
+		for (int q = 0; q < 7; q++) {
+			if (AA && B.even)
+				f[q] = K::ld(cur + q * A.XYZ + s0, q, pr, sc);
+			else if (AA)
+				f[q] = K::ld(cur + opp(q) * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), pr, sc);
+			else
+				f[q] = K::ld(cur + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, pr, sc);
+		}
+		switch (gi) {
+			case A_INFLOW:
+				phi = (real) B.inflow_phi;
+				K::set_eq(f, phi, vx, vy, vz);
+				break;
+			case A_WALL:
+				phi = K::density(f);
+#pragma unroll
+				for (int q = 1; q < 7; q += 2) {
+					const real t = f[q];
+					f[q] = f[q + 1];
+					f[q + 1] = t;
+				}
+				break;
+			case A_SYM_TOP: f[zzm] = f[zzp]; phi = K::density(f); break;
+			case A_SYM_BOTTOM: f[zzp] = f[zzm]; phi = K::density(f); break;
+			case A_SYM_LEFT: f[pzz] = f[mzz]; phi = K::density(f); break;
+			case A_SYM_RIGHT: f[mzz] = f[pzz]; phi = K::density(f); break;
+			case A_SYM_BACK: f[zpz] = f[zmz]; phi = K::density(f); break;
+			case A_SYM_FRONT: f[zmz] = f[zpz]; phi = K::density(f); break;
+			default: phi = K::density(f); break;
+		}
+		if (gi == A_FLUID || gi == A_PERIODIC || gi == A_SOLID || gi == A_OUTFLOW_RIGHT) {
+			K::collide(B.collision, f, phi, vx, vy, vz, D);
+			if (B.source) {
+				const real S = macro_at(B.source);
+#pragma unroll
+				for (int q = 0; q < 7; q++)
+					f[q] += (real) w7(q) * S;
+			}
+		}
+		elem* out = (elem*) (AA ? B.df0 : B.df1);
+#pragma unroll
+		for (int q = 0; q < 7; q++) {
+			if (AA && B.even)
+				K::st(out + opp(q) * A.XYZ + s0, opp(q), f[q], pr, sc);
+			else if (AA)
+				K::st(out + q * A.XYZ + sidx(NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), q, f[q], pr, sc);
+			else
+				K::st(out + q * A.XYZ + s0, q, f[q], pr, sc);
+		}
+	}
+	if (A.out_macro && B.macro) {
+		if (A.macro_f64)
+			((double*) B.macro)[s0] = phi;
+		else
+			((float*) B.macro)[s0] = (float) phi;
+	}
+}
+
+template <int ST, int COLL, int PATTERN>
+__global__ void __launch_bounds__(256) k_nse_ade(const StepArgs A, const AdeArgs B)
+{
+	int64_t x, z, y;
+	if (! block_coords(A, x, z, y))
+		return;
+	typename Storage<ST>::real vx, vy, vz;
+	Nb NX, NY, NZ;
+	General<ST, COLL, PATTERN>::site(A, x, y, z, vx, vy, vz, NX, NY, NZ);
+	const int64_t s0 = (NX.o * A.Zs + NZ.o) * A.Ys + NY.o;
+	ade_site<ST, PATTERN>(A, B, s0, vx, vy, vz, NX, NY, NZ);
+}
+
+template <int ST>
+__global__ void k_ade_set_equilibrium(void* df0, void* df1, int ncopies, int64_t XYZ, int64_t n, const void* phi,
+									  int phi_f64, double phi0, double pr, double sc)
+{
+	using K = Ade7<ST>;
+	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n)
+		return;
+	const double p = phi ? (phi_f64 ? ((const double*) phi)[s] : (double) ((const float*) phi)[s]) : phi0;
+	typename K::real f[7];
+	K::set_eq(f, (typename K::real) p, 0, 0, 0);
+	for (int k = 0; k < ncopies; k++) {
+		auto* out = (typename K::elem*) (k == 0 ? df0 : df1);
+#pragma unroll
+		for (int q = 0; q < 7; q++)
+			K::st(out + q * XYZ + s, q, f[q], (typename K::real) pr, (typename K::real) sc);
+	}
+}
+
+template <int ST>
+__global__ void k_ade_export(const void* df, double* out, int64_t XYZ, int64_t n, double pr, double sc)
+{
+	using K = Ade7<ST>;
+	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n)
+		return;
+#pragma unroll
+	for (int q = 0; q < 7; q++)
+		out[q * XYZ + s] = (double) K::ld((const typename K::elem*) df + q * XYZ + s, q, (typename K::real) pr, (typename K::real) sc);
+}
+
+// ===========================================================================
+// Initialisation, transfer and reduction kernels
+// ===========================================================================
+struct FieldArgs
+{
+	void* df[2];
+	int ncopies;
+	int64_t XYZ;
+	int64_t nsites;  // stored sites incl. overlaps
+	const double* rho;
+	const double* vx;
+	const double* vy;
+	const double* vz;
+	double c[4];
+};
+
+template <int ST, int EQK>
+__global__ void k_set_equilibrium(const FieldArgs A)
+{
+	using S = Storage<ST>;
+	using real = typename S::real;
+	using P = Phys<ST>;
+	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= A.nsites)
+		return;
+	const real rho = (real) (A.rho ? A.rho[s] : A.c[0]);
+	const real vx = (real) (A.vx ? A.vx[s] : A.c[1]);
+	const real vy = (real) (A.vy ? A.vy[s] : A.c[2]);
+	const real vz = (real) (A.vz ? A.vz[s] : A.c[3]);
+	real f[27];
+	General<ST, EQK == 0 ? TNL_LBM_COLL_CUM : TNL_LBM_COLL_MRT_LES, TNL_LBM_PATTERN_AA>::set_eq(f, rho, vx, vy, vz);
+	for (int k = 0; k < A.ncopies; k++) {
+		auto* out = (typename S::elem*) A.df[k];
+#pragma unroll
+		for (int q = 0; q < 27; q++)
+			S::st(out + q * A.XYZ + s, f[q]);
+	}
+}
+
+struct MacroArgs
+{
+	const void* df;
+	void* macro;
+	int macro_f64;
+	int64_t XYZ;
+	int64_t X, Y, Z, ox, oy, oz, Ys, Zs;
+};
+
+// LBM_BLOCK::computeInitialMacro (lbm_block.hpp:281-305); forces zeroed
+template <int ST>
+__global__ void k_compute_macro(const MacroArgs A)
+{
+	using S = Storage<ST>;
+	using real = typename S::real;
+	const int64_t n = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (n >= A.X * A.Y * A.Z)
+		return;
+	const int64_t y = n % A.Y, t = n / A.Y, z = t % A.Z, x = t / A.Z;
+	const int64_t s = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys + (y + A.oy);
+	real f[27];
+#pragma unroll
+	for (int q = 0; q < 27; q++)
+		f[q] = S::ld((const typename S::elem*) A.df + q * A.XYZ + s);
+	real rho, vx, vy, vz;
+	Phys<ST>::density_velocity(f, 0, 0, 0, rho, vx, vy, vz);
+	if (A.macro_f64) {
+		double* m = (double*) A.macro;
+		m[s] = rho; m[A.XYZ + s] = vx; m[2 * A.XYZ + s] = vy; m[3 * A.XYZ + s] = vz;
+	}
+	else {
+		float* m = (float*) A.macro;
+		m[s] = rho; m[A.XYZ + s] = vx; m[2 * A.XYZ + s] = vy; m[3 * A.XYZ + s] = vz;
+	}
+}
+
+template <int ST, int Q, bool EXPORT>
+__global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites)
+{
+	using S = Storage<ST>;
+	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nsites)
+		return;
+	auto* p = (typename S::elem*) df;
+#pragma unroll
+	for (int q = 0; q < Q; q++) {
+		const double w = S::dev ? (Q == 27 ? w27(q) : (q == 0 ? 0.25 : 0.125)) : 0.0;
+		if constexpr (EXPORT)
+			buf[q * XYZ + s] = (double) S::ld(p + q * XYZ + s) + w;
+		else
+			S::st(p + q * XYZ + s, (typename S::real) (buf[q * XYZ + s] - w));
+	}
+}
+
+// FP64 reductions over the macro field: deterministic two-pass block sums
+constexpr int RED_N = 6;
+constexpr int RED_BLOCKS = 1024;
+
+__device__ void block_sum(double* v, double (*sh)[RED_N])
+{
+	const int t = threadIdx.x;
+#pragma unroll
+	for (int i = 0; i < RED_N; i++)
+		sh[t][i] = v[i];
+	__syncthreads();
+	for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+		if (t < w)
+#pragma unroll
+			for (int i = 0; i < RED_N; i++)
+				sh[t][i] += sh[t + w][i];
+		__syncthreads();
+	}
+}
+
+__global__ void __launch_bounds__(256) k_reduce_macro(const MacroArgs A, double* partial)
+{
+	__shared__ double sh[256][RED_N];
+	double v[RED_N] = {0, 0, 0, 0, 0, 0};
+	const int64_t N = A.X * A.Y * A.Z;
+	for (int64_t n = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (int64_t) gridDim.x * blockDim.x) {
+		const int64_t y = n % A.Y, t = n / A.Y, z = t % A.Z, x = t / A.Z;
+		const int64_t s = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys + (y + A.oy);
+		double rho, vx, vy, vz;
+		if (A.macro_f64) {
+			const double* m = (const double*) A.macro;
+			rho = m[s]; vx = m[A.XYZ + s]; vy = m[2 * A.XYZ + s]; vz = m[3 * A.XYZ + s];
+		}
+		else {
+			const float* m = (const float*) A.macro;
+			rho = m[s]; vx = m[A.XYZ + s]; vy = m[2 * A.XYZ + s]; vz = m[3 * A.XYZ + s];
+		}
+		const double u2 = vx * vx + vy * vy + vz * vz;
+		v[0] += rho;
+		v[1] += rho * u2;
+		v[2] += (rho != rho) ? 1.0 : 0.0;
+		v[3] += (rho - 1.0) * (rho - 1.0);
+		v[4] += u2;
+		v[5] += 1.0;
+	}
+	block_sum(v, sh);
+	if (threadIdx.x == 0)
+#pragma unroll
+		for (int i = 0; i < RED_N; i++)
+			partial[blockIdx.x * RED_N + i] = sh[0][i];
+}
+
+__global__ void __launch_bounds__(256) k_reduce_final(const double* partial, int nparts, double* out)
+{
+	__shared__ double sh[256][RED_N];
+	double v[RED_N] = {0, 0, 0, 0, 0, 0};
+	for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+#pragma unroll
+		for (int i = 0; i < RED_N; i++)
+			v[i] += partial[b * RED_N + i];
+	block_sum(v, sh);
+	if (threadIdx.x == 0)
+#pragma unroll
+		for (int i = 0; i < RED_N; i++)
+			out[i] = sh[0][i];
+}
+
+// ===========================================================================
+// Host side: validation and dispatch
+// ===========================================================================
+int validate(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
+{
+	if (! cfg || ! SD)
+		return fail(TNL_LBM_ERR_NULL, "null config or data pointer");
+	if (cfg->struct_size != sizeof(tnl_lbm_config) || cfg->version != TNL_LBM_ABI_VERSION)
+		return fail(TNL_LBM_ERR_VERSION, "tnl_lbm_config struct_size/version mismatch");
+	if (SD->struct_size != sizeof(tnl_lbm_d3q27_data) || SD->version != TNL_LBM_ABI_VERSION)
+		return fail(TNL_LBM_ERR_VERSION, "tnl_lbm_d3q27_data struct_size/version mismatch");
+	if (SD->X <= 0 || SD->Y <= 0 || SD->Z <= 0 || SD->ox < 0 || SD->oy < 0 || SD->oz < 0)
+		return fail(TNL_LBM_ERR_DIMS, "lattice sizes must be positive and overlaps non-negative");
+	const int64_t stored = (SD->X + 2 * SD->ox) * (SD->Y + 2 * SD->oy) * (SD->Z + 2 * SD->oz);
+	if (SD->XYZ < stored)
+		return fail(TNL_LBM_ERR_DIMS, "XYZ is smaller than the stored lattice");
+	if (SD->Y + 2 * SD->oy > (int64_t) 1 << 30)
+		return fail(TNL_LBM_ERR_DIMS, "Y extent too large");
+	if (cfg->storage < TNL_LBM_STORE_F64 || cfg->storage > TNL_LBM_STORE_BF16_DEV)
+		return fail(TNL_LBM_ERR_ARG, "unknown storage form");
+	if (cfg->pattern != TNL_LBM_PATTERN_AA && cfg->pattern != TNL_LBM_PATTERN_AB)
+		return fail(TNL_LBM_ERR_ARG, "unknown streaming pattern");
+	if (cfg->macro_real != TNL_LBM_REAL_F64 && cfg->macro_real != TNL_LBM_REAL_F32)
+		return fail(TNL_LBM_ERR_ARG, "unknown macro element type");
+	if (cfg->collision < TNL_LBM_COLL_CUM || cfg->collision > TNL_LBM_COLL_CUM_2017_ANTIALIAS)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "collision operator not built into this library");
+	if (cfg->storage >= TNL_LBM_STORE_F32_DEV && cfg->collision != TNL_LBM_COLL_CUM)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "deviation storage is built for the default cumulant only");
+	if (! SD->df[0] || (cfg->pattern == TNL_LBM_PATTERN_AB && ! SD->df[1]))
+		return fail(TNL_LBM_ERR_NULL, "population array missing");
+	return TNL_LBM_OK;
+}
+
+int hip_check(hipError_t e, const char* what)
+{
+	if (e != hipSuccess)
+		return fail(TNL_LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+	return TNL_LBM_OK;
+}
+
+// omega1 = 1/(3 nu + 1/2) in the arithmetic of the storage form (col_cum.h:195)
+double omega1_of(int storage, double nu)
+{
+	if (storage == TNL_LBM_STORE_F64)
+		return 1.0 / (3.0 * nu + 0.5);
+	volatile float n = (float) nu;
+	volatile float t = 3.0f * n;
+	volatile float d = t + 0.5f;
+	return (double) (1.0f / d);
+}
+
+template <int ST, int COLL>
+int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
+{
+	if (! A.map && cfg->pattern == TNL_LBM_PATTERN_AA) {
+		if (A.even)
+			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(grid), dim3(threads), 0, st, A);
+		else
+			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(grid), dim3(threads), 0, st, A);
+	}
+	else if (cfg->pattern == TNL_LBM_PATTERN_AA)
+		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, A);
+	else
+		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, A);
+	return hip_check(hipGetLastError(), "step launch");
+}
+
+template <int ST>
+int dispatch_coll(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
+{
+	if constexpr (Storage<ST>::dev)
+		return launch_step<ST, TNL_LBM_COLL_CUM>(cfg, A, threads, grid, st);
+	else {
+		switch (cfg->collision) {
+			case TNL_LBM_COLL_CUM: return launch_step<ST, TNL_LBM_COLL_CUM>(cfg, A, threads, grid, st);
+			case TNL_LBM_COLL_CUM_2017: return launch_step<ST, TNL_LBM_COLL_CUM_2017>(cfg, A, threads, grid, st);
+			default: return launch_step<ST, TNL_LBM_COLL_CUM_2017_ANTIALIAS>(cfg, A, threads, grid, st);
+		}
+	}
+}
+
+#define STORAGE_SWITCH(storage, EXPR)                                                      \
+	switch (storage) {                                                                     \
+		case TNL_LBM_STORE_F64: { constexpr int ST = TNL_LBM_STORE_F64; return EXPR; }         \
+		case TNL_LBM_STORE_F32: { constexpr int ST = TNL_LBM_STORE_F32; return EXPR; }         \
+		case TNL_LBM_STORE_F32_DEV: { constexpr int ST = TNL_LBM_STORE_F32_DEV; return EXPR; } \
+		case TNL_LBM_STORE_F16_DEV: { constexpr int ST = TNL_LBM_STORE_F16_DEV; return EXPR; } \
+		default: { constexpr int ST = TNL_LBM_STORE_BF16_DEV; return EXPR; }                   \
+	}
+
+int64_t stored_sites(const tnl_lbm_d3q27_data* SD)
+{
+	return (SD->X + 2 * SD->ox) * (SD->Y + 2 * SD->oy) * (SD->Z + 2 * SD->oz);
+}
+
+MacroArgs macro_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* df)
+{
+	MacroArgs M;
+	M.df = df;
+	M.macro = SD->macro;
+	M.macro_f64 = cfg->macro_real == TNL_LBM_REAL_F64;
+	M.XYZ = SD->XYZ;
+	M.X = SD->X; M.Y = SD->Y; M.Z = SD->Z;
+	M.ox = SD->ox; M.oy = SD->oy; M.oz = SD->oz;
+	M.Ys = SD->Y + 2 * SD->oy;
+	M.Zs = SD->Z + 2 * SD->oz;
+	return M;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+uint32_t tnl_lbm_abi_version(void)
+{
+	return TNL_LBM_ABI_VERSION;
+}
+
+const char* tnl_lbm_last_error(void)
+{
+	return g_err.c_str();
+}
+
+static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t offset[3],
+						  const int64_t end[3], uint32_t flags, StepArgs& A, unsigned& threads, unsigned& grid);
+
+int tnl_lbm_d3q27_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t offset[3],
+					   const int64_t end[3], uint32_t flags, void* stream)
+{
+	StepArgs A;
+	unsigned threads = 0, grid = 0;
+	if (int rc = make_step_args(cfg, SD, offset, end, flags, A, threads, grid))
+		return rc;
+	if (grid == 0)
+		return TNL_LBM_OK;
+	hipStream_t st = (hipStream_t) stream;
+	STORAGE_SWITCH(cfg->storage, (dispatch_coll<ST>(cfg, A, threads, grid, st)));
+}
+
+static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t offset[3],
+						  const int64_t end[3], uint32_t flags, StepArgs& A, unsigned& threads, unsigned& grid)
+{
+	grid = 0;
+	if (int rc = validate(cfg, SD))
+		return rc;
+	if (! offset || ! end)
+		return fail(TNL_LBM_ERR_NULL, "offset/end missing");
+	const int64_t lo[3] = {SD->X, SD->Y, SD->Z};
+	for (int d = 0; d < 3; d++)
+		if (offset[d] < 0 || end[d] > lo[d] || end[d] < offset[d])
+			return fail(TNL_LBM_ERR_DIMS, "offset/end outside the local lattice");
+	if ((flags & TNL_LBM_FLAG_OUTPUT_MACRO) && ! SD->macro)
+		return fail(TNL_LBM_ERR_NULL, "output_macro requested without a macro array");
+	std::memset(&A, 0, sizeof(A));
+	A.df0 = SD->df[0];
+	A.df1 = SD->df[1];
+	A.macro = SD->macro;
+	A.map = SD->map;
+	A.XYZ = SD->XYZ;
+	A.X = SD->X; A.Y = SD->Y; A.Z = SD->Z;
+	A.ox = SD->ox; A.oy = SD->oy; A.oz = SD->oz;
+	A.Ys = SD->Y + 2 * SD->oy;
+	A.Zs = SD->Z + 2 * SD->oz;
+	A.offx = (int32_t) offset[0]; A.offy = (int32_t) offset[1]; A.offz = (int32_t) offset[2];
+	A.nx = (int32_t) (end[0] - offset[0]); A.ny = (int32_t) (end[1] - offset[1]); A.nz = (int32_t) (end[2] - offset[2]);
+	if (A.nx == 0 || A.ny == 0 || A.nz == 0)
+		return TNL_LBM_OK;
+	threads = A.ny > 128 ? 256 : (A.ny > 64 ? 128 : 64);
+	A.nyb = (uint32_t) ((A.ny + threads - 1) / threads);
+	const uint64_t nb = (uint64_t) A.nyb * A.nz * A.nx;
+	if (nb >= (1ull << 31))
+		return fail(TNL_LBM_ERR_DIMS, "lattice box too large for one launch");
+	A.nblocks = (uint32_t) nb;
+	const uint32_t padded = (A.nblocks + NXCD - 1) / NXCD * NXCD;
+	A.per_xcd = padded / NXCD;
+	A.even = SD->even_iter ? 1 : 0;
+	A.out_macro = (flags & TNL_LBM_FLAG_OUTPUT_MACRO) ? 1 : 0;
+	A.macro_f64 = cfg->macro_real == TNL_LBM_REAL_F64;
+	A.dist_x = (flags & TNL_LBM_FLAG_DISTRIBUTED_X) != 0;
+	A.dist_y = (flags & TNL_LBM_FLAG_DISTRIBUTED_Y) != 0;
+	A.dist_z = (flags & TNL_LBM_FLAG_DISTRIBUTED_Z) != 0;
+	// a distributed axis reads its neighbours from the overlap layer, which must exist
+	if ((A.dist_x && SD->ox < 1) || (A.dist_y && SD->oy < 1) || (A.dist_z && SD->oz < 1))
+		return fail(TNL_LBM_ERR_DIMS, "a distributed axis needs an overlap of at least one site");
+	if (SD->lbm_viscosity == 0)
+		return fail(TNL_LBM_ERR_ARG, "LBM viscosity is 0");  // state.hpp:1240-1245
+	A.omega1 = omega1_of(cfg->storage, SD->lbm_viscosity);
+	A.fx = SD->fx; A.fy = SD->fy; A.fz = SD->fz;
+	A.inflow[0] = SD->inflow_vx; A.inflow[1] = SD->inflow_vy; A.inflow[2] = SD->inflow_vz;
+	for (int k = 0; k < 3; k++)
+		A.prof[k] = SD->inflow_profile[k];
+	grid = padded;
+	return TNL_LBM_OK;
+}
+
+static int validate_ade(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* ADE)
+{
+	if (! cfg || ! ADE)
+		return fail(TNL_LBM_ERR_NULL, "null config or ADE data");
+	if (ADE->struct_size != sizeof(tnl_lbm_d3q7_data) || ADE->version != TNL_LBM_ABI_VERSION)
+		return fail(TNL_LBM_ERR_VERSION, "tnl_lbm_d3q7_data struct_size/version mismatch");
+	if (! ADE->df[0] || (cfg->pattern == TNL_LBM_PATTERN_AB && ! ADE->df[1]))
+		return fail(TNL_LBM_ERR_NULL, "ADE population array missing");
+	if (ADE->collision < TNL_LBM_ADE_COLL_SRT || ADE->collision > TNL_LBM_ADE_COLL_CLBM)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "unknown ADE collision");
+	if (cfg->storage >= TNL_LBM_STORE_F32_DEV && ADE->store_scale == 0)
+		return fail(TNL_LBM_ERR_ARG, "ADE store_scale must be non-zero");
+	return TNL_LBM_OK;
+}
+
+int tnl_lbm_nse_ade_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* nse, const tnl_lbm_d3q7_data* ade,
+						 const int64_t offset[3], const int64_t end[3], uint32_t flags, void* stream)
+{
+	StepArgs A;
+	unsigned threads = 0, grid = 0;
+	if (int rc = make_step_args(cfg, nse, offset, end, flags, A, threads, grid))
+		return rc;
+	if (int rc = validate_ade(cfg, ade))
+		return rc;
+	if (ade->X != nse->X || ade->Y != nse->Y || ade->Z != nse->Z || ade->ox != nse->ox || ade->oy != nse->oy
+		|| ade->oz != nse->oz || ade->XYZ != nse->XYZ)
+		return fail(TNL_LBM_ERR_DIMS, "NSE and ADE lattices must have the same geometry");
+	if (cfg->collision != TNL_LBM_COLL_CUM)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the coupled kernel is built for the default cumulant");
+	if ((flags & TNL_LBM_FLAG_OUTPUT_MACRO) && ! ade->macro)
+		return fail(TNL_LBM_ERR_NULL, "output_macro requested without an ADE macro array");
+	if (grid == 0)
+		return TNL_LBM_OK;
+	AdeArgs B;
+	std::memset(&B, 0, sizeof(B));
+	B.df0 = ade->df[0];
+	B.df1 = ade->df[1];
+	B.macro = ade->macro;
+	B.map = ade->map;
+	B.diff = ade->diffusion_coefficient;
+	B.source = ade->source;
+	B.D = ade->lbm_diffusivity;
+	B.inflow_phi = ade->inflow_phi;
+	B.phi_ref = ade->phi_ref;
+	B.scale = ade->store_scale;
+	B.collision = ade->collision;
+	B.even = ade->even_iter ? 1 : 0;
+	hipStream_t st = (hipStream_t) stream;
+	auto go = [&](auto stc) {
+		constexpr int ST = decltype(stc)::value;
+		if (cfg->pattern == TNL_LBM_PATTERN_AA)
+			hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, A, B);
+		else
+			hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, A, B);
+		return hip_check(hipGetLastError(), "nse_ade launch");
+	};
+	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));
+}
+
+int tnl_lbm_d3q7_set_equilibrium(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* ADE, const void* phi, double phi0,
+								 void* stream)
+{
+	if (int rc = validate_ade(cfg, ADE))
+		return rc;
+	const int64_t n = (ADE->X + 2 * ADE->ox) * (ADE->Y + 2 * ADE->oy) * (ADE->Z + 2 * ADE->oz);
+	const unsigned grid = (unsigned) ((n + 255) / 256);
+	const int ncop = cfg->pattern == TNL_LBM_PATTERN_AB ? 2 : 1;
+	hipStream_t st = (hipStream_t) stream;
+	auto go = [&](auto stc) {
+		constexpr int ST = decltype(stc)::value;
+		hipLaunchKernelGGL((k_ade_set_equilibrium<ST>), dim3(grid), dim3(256), 0, st, ADE->df[0], ADE->df[1], ncop, ADE->XYZ, n, phi,
+						   cfg->macro_real == TNL_LBM_REAL_F64, phi0, ADE->phi_ref, ADE->store_scale);
+		return hip_check(hipGetLastError(), "d3q7 set_equilibrium launch");
+	};
+	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));
+}
+
+int tnl_lbm_d3q7_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* ADE, int which, double* out, void* stream)
+{
+	if (int rc = validate_ade(cfg, ADE))
+		return rc;
+	if (which < 0 || which > 1 || ! ADE->df[which] || ! out)
+		return fail(TNL_LBM_ERR_ARG, "bad array selector or buffer");
+	const int64_t n = (ADE->X + 2 * ADE->ox) * (ADE->Y + 2 * ADE->oy) * (ADE->Z + 2 * ADE->oz);
+	const unsigned grid = (unsigned) ((n + 255) / 256);
+	hipStream_t st = (hipStream_t) stream;
+	auto go = [&](auto stc) {
+		constexpr int ST = decltype(stc)::value;
+		hipLaunchKernelGGL((k_ade_export<ST>), dim3(grid), dim3(256), 0, st, ADE->df[which], out, ADE->XYZ, n, ADE->phi_ref,
+						   ADE->store_scale);
+		return hip_check(hipGetLastError(), "d3q7 export launch");
+	};
+	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));
+}
+
+static int set_eq_impl(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const double* rho, const double* vx,
+					   const double* vy, const double* vz, const double c[4], void* stream)
+{
+	if (int rc = validate(cfg, SD))
+		return rc;
+	FieldArgs F;
+	F.df[0] = SD->df[0];
+	F.df[1] = SD->df[1];
+	F.ncopies = cfg->pattern == TNL_LBM_PATTERN_AB ? 2 : 1;
+	F.XYZ = SD->XYZ;
+	F.nsites = stored_sites(SD);
+	F.rho = rho; F.vx = vx; F.vy = vy; F.vz = vz;
+	for (int i = 0; i < 4; i++)
+		F.c[i] = c[i];
+	const unsigned grid = (unsigned) ((F.nsites + 255) / 256);
+	hipStream_t st = (hipStream_t) stream;
+	const bool cum = cfg->collision <= TNL_LBM_COLL_CUM_2017_ANTIALIAS;
+	auto go = [&](auto stc) {
+		constexpr int ST = decltype(stc)::value;
+		if (cum)
+			hipLaunchKernelGGL((k_set_equilibrium<ST, 0>), dim3(grid), dim3(256), 0, st, F);
+		else
+			hipLaunchKernelGGL((k_set_equilibrium<ST, 1>), dim3(grid), dim3(256), 0, st, F);
+		return hip_check(hipGetLastError(), "set_equilibrium launch");
+	};
+	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));
+}
+
+int tnl_lbm_d3q27_set_equilibrium(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, double rho, double vx,
+								  double vy, double vz, void* stream)
+{
+	const double c[4] = {rho, vx, vy, vz};
+	return set_eq_impl(cfg, SD, nullptr, nullptr, nullptr, nullptr, c, stream);
+}
+
+int tnl_lbm_d3q27_set_equilibrium_field(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const double* rho,
+										const double* vx, const double* vy, const double* vz, void* stream)
+{
+	if (! rho || ! vx || ! vy || ! vz)
+		return fail(TNL_LBM_ERR_NULL, "field pointer missing");
+	const double c[4] = {0, 0, 0, 0};
+	return set_eq_impl(cfg, SD, rho, vx, vy, vz, c, stream);
+}
+
+int tnl_lbm_d3q27_compute_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, void* stream)
+{
+	if (int rc = validate(cfg, SD))
+		return rc;
+	if (! SD->macro)
+		return fail(TNL_LBM_ERR_NULL, "macro array missing");
+	const MacroArgs M = macro_args(cfg, SD, SD->df[0]);
+	const unsigned grid = (unsigned) ((SD->X * SD->Y * SD->Z + 255) / 256);
+	hipStream_t st = (hipStream_t) stream;
+	auto go = [&](auto stc) {
+		constexpr int ST = decltype(stc)::value;
+		hipLaunchKernelGGL((k_compute_macro<ST>), dim3(grid), dim3(256), 0, st, M);
+		return hip_check(hipGetLastError(), "compute_macro launch");
+	};
+	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));
+}
+
+static int transfer(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int which, double* buf, bool exp, void* stream)
+{
+	if (int rc = validate(cfg, SD))
+		return rc;
+	if (which < 0 || which > 1 || ! SD->df[which] || ! buf)
+		return fail(TNL_LBM_ERR_ARG, "bad array selector or buffer");
+	const int64_t n = stored_sites(SD);
+	const unsigned grid = (unsigned) ((n + 255) / 256);
+	hipStream_t st = (hipStream_t) stream;
+	auto go = [&](auto stc) {
+		constexpr int ST = decltype(stc)::value;
+		if (exp)
+			hipLaunchKernelGGL((k_transfer<ST, 27, true>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n);
+		else
+			hipLaunchKernelGGL((k_transfer<ST, 27, false>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n);
+		return hip_check(hipGetLastError(), "transfer launch");
+	};
+	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));
+}
+
+int tnl_lbm_d3q27_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int which, double* out, void* stream)
+{
+	return transfer(cfg, SD, which, out, true, stream);
+}
+
+int tnl_lbm_d3q27_import_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int which, const double* in, void* stream)
+{
+	return transfer(cfg, SD, which, const_cast<double*>(in), false, stream);
+}
+
+int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, double out[6], void* stream)
+{
+	if (int rc = validate(cfg, SD))
+		return rc;
+	if (! SD->macro || ! out)
+		return fail(TNL_LBM_ERR_NULL, "macro array or output missing");
+	// per-device partial-sum buffer, allocated once
+	static thread_local int dev_cached = -1;
+	static thread_local double* partial = nullptr;
+	int dev = 0;
+	if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice"))
+		return rc;
+	if (dev != dev_cached) {
+		if (int rc = hip_check(hipMalloc(&partial, sizeof(double) * (RED_BLOCKS * RED_N + RED_N)), "hipMalloc reduce buffer"))
+			return rc;
+		dev_cached = dev;
+	}
+	const MacroArgs M = macro_args(cfg, SD, SD->df[0]);
+	hipStream_t st = (hipStream_t) stream;
+	hipLaunchKernelGGL(k_reduce_macro, dim3(RED_BLOCKS), dim3(256), 0, st, M, partial);
+	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*) partial, RED_BLOCKS, partial + RED_BLOCKS * RED_N);
+	if (int rc = hip_check(hipGetLastError(), "reduce launch"))
+		return rc;
+	if (int rc = hip_check(hipMemcpyAsync(out, partial + RED_BLOCKS * RED_N, sizeof(double) * RED_N, hipMemcpyDeviceToHost, st), "reduce copy"))
+		return rc;
+	return hip_check(hipStreamSynchronize(st), "reduce sync");
+}
+
+}  // extern "C"

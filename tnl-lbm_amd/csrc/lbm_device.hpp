@@ -1,0 +1,552 @@
+// Device-side lattice physics for gfx950: direction tables, storage forms and
+// the per-site plugins (density/velocity, equilibria, cumulant collisions).
+//
+// One site per lane; everything is held in registers.  Reference behaviour:
+// TNL-LBM include/lbm3d/d3q27/{common.h, eq_inv_cum.h, eq.h, col_cum.h} and,
+// for the deviation storage forms, the well-conditioned family
+// d3q27/{common_well.h, eq_well.h, col_cum_well.h}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tnl_lbm_amd.h"
+
+#define LBM_DEV __device__ __forceinline__
+
+namespace lbm {
+
+// ---------------------------------------------------------------------------
+// Direction numbering of defs.h:317-350.  Indexed with compile-time q only
+// (all loops over q are fully unrolled).
+enum : int
+{
+	zzz = 0, pzz = 1, mzz = 2, zpz = 3, zmz = 4, zzp = 5, zzm = 6,
+	ppz = 7, mmz = 8, pmz = 9, mpz = 10, pzp = 11, mzm = 12, pzm = 13, mzp = 14,
+	zpp = 15, zmm = 16, zpm = 17, zmp = 18,
+	ppp = 19, mmm = 20, ppm = 21, mmp = 22, pmp = 23, mpm = 24, pmm = 25, mpp = 26
+};
+constexpr int CX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+constexpr int CY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1};
+constexpr int CZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int opp(int q)
+{
+	return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1);
+}
+// index of the direction (a-1, b-1, c-1)
+constexpr int qidx(int a, int b, int c)
+{
+	int r = -1;
+	for (int q = 0; q < 27; q++)
+		if (CX[q] == a - 1 && CY[q] == b - 1 && CZ[q] == c - 1)
+			r = q;
+	return r;
+}
+// 1-D rest weights W(-1)=1/6, W(0)=2/3, W(1)=1/6 and the D3Q27 weights
+constexpr double W1[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+constexpr double w27(int q)
+{
+	return W1[CX[q] + 1] * W1[CY[q] + 1] * W1[CZ[q] + 1];
+}
+
+// ---------------------------------------------------------------------------
+// Storage forms.  `real` is the arithmetic type, `elem` the stored type.
+template <int ST>
+struct Storage;
+template <>
+struct Storage<TNL_LBM_STORE_F64>
+{
+	using elem = double;
+	using real = double;
+	static constexpr bool dev = false;
+	static LBM_DEV real ld(const elem* p) { return *p; }
+	static LBM_DEV void st(elem* p, real v) { *p = v; }
+};
+template <>
+struct Storage<TNL_LBM_STORE_F32>
+{
+	using elem = float;
+	using real = float;
+	static constexpr bool dev = false;
+	static LBM_DEV real ld(const elem* p) { return *p; }
+	static LBM_DEV void st(elem* p, real v) { *p = v; }
+};
+template <>
+struct Storage<TNL_LBM_STORE_F32_DEV>
+{
+	using elem = float;
+	using real = float;
+	static constexpr bool dev = true;
+	static LBM_DEV real ld(const elem* p) { return *p; }
+	static LBM_DEV void st(elem* p, real v) { *p = v; }
+};
+template <>
+struct Storage<TNL_LBM_STORE_F16_DEV>
+{
+	using elem = _Float16;
+	using real = float;
+	static constexpr bool dev = true;
+	static LBM_DEV real ld(const elem* p) { return (float) *p; }
+	static LBM_DEV void st(elem* p, real v) { *p = (_Float16) v; }
+};
+template <>
+struct Storage<TNL_LBM_STORE_BF16_DEV>
+{
+	using elem = __bf16;
+	using real = float;
+	static constexpr bool dev = true;
+	static LBM_DEV real ld(const elem* p) { return (float) *p; }
+	static LBM_DEV void st(elem* p, real v) { *p = (__bf16) v; }
+};
+
+// ---------------------------------------------------------------------------
+// Absolute form (F64 and F32 storage): the reference plugins.
+template <typename real>
+struct Abs
+{
+	// D3Q27_COMMON::computeDensityAndVelocity (common.h:17-50), same grouping
+	static LBM_DEV void density_velocity(const real* f, real Fx, real Fy, real Fz, real& rho, real& vx, real& vy, real& vz)
+	{
+		rho = ((((f[ppp] + f[mmm]) + (f[pmp] + f[mpm])) + ((f[ppm] + f[mmp]) + (f[mpp] + f[pmm])))
+			   + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[pzp] + f[mzm]) + (f[pzm] + f[mzp]))
+				  + ((f[ppz] + f[mmz]) + (f[pmz] + f[mpz])))
+			   + ((f[pzz] + f[mzz]) + (f[zpz] + f[zmz]) + (f[zzp] + f[zzm])))
+			+ f[zzz];
+		const real rinv = (real) 1 / rho;
+		vz = ((((f[ppp] - f[mmm]) + (f[mpp] - f[pmm])) + ((f[pmp] - f[mpm]) + (f[mmp] - f[ppm])))
+			  + (((f[zpp] - f[zmm]) + (f[zmp] - f[zpm])) + ((f[pzp] - f[mzm]) + (f[mzp] - f[pzm]))) + (f[zzp] - f[zzm])
+			  + Fz * (real) 0.5)
+			/ rho;
+		vx = ((((f[ppp] - f[mmm]) + (f[pmp] - f[mpm])) + ((f[ppm] - f[mmp]) + (f[pmm] - f[mpp])))
+			  + (((f[pzp] - f[mzm]) + (f[pzm] - f[mzp])) + ((f[ppz] - f[mmz]) + (f[pmz] - f[mpz]))) + (f[pzz] - f[mzz])
+			  + Fx * (real) 0.5)
+			/ rho;
+		vy = ((((f[ppp] - f[mmm]) + (f[ppm] - f[mmp])) + ((f[mpp] - f[pmm]) + (f[mpm] - f[pmp])))
+			  + (((f[ppz] - f[mmz]) + (f[mpz] - f[pmz])) + ((f[zpp] - f[zmm]) + (f[zpm] - f[zmp]))) + (f[zpz] - f[zmz])
+			  + Fy * (real) 0.5)
+			/ rho;
+		(void) rinv;
+	}
+
+	// D3Q27_EQ_INV_CUM (eq_inv_cum.h:183-295): rho * W(vx) W(vy) W(vz)
+	template <int c>
+	static LBM_DEV real invcum_factor(real v)
+	{
+		if constexpr (c == 0)
+			return (real) 3 * v * v - (real) 2;
+		else if constexpr (c > 0)
+			return (real) 3 * v * v + (real) 3 * v + (real) 1;
+		else
+			return (real) 3 * v * v - (real) 3 * v + (real) 1;
+	}
+	template <int q>
+	static LBM_DEV real eq_invcum(real rho, real vx, real vy, real vz)
+	{
+		constexpr int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+		constexpr double coef = nz == 0 ? -1.0 / 27.0 : nz == 1 ? 1.0 / 54.0 : nz == 2 ? -1.0 / 108.0 : 1.0 / 216.0;
+		return (real) coef * rho * (invcum_factor<CX[q]>(vx) * invcum_factor<CY[q]>(vy) * invcum_factor<CZ[q]>(vz));
+	}
+	// D3Q27_EQ (eq.h:13-130)
+	template <int q>
+	static LBM_DEV real eq_poly(real rho, real vx, real vy, real vz)
+	{
+		constexpr int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+		constexpr double w = nz == 0 ? 8.0 / 27.0 : nz == 1 ? 2.0 / 27.0 : nz == 2 ? 1.0 / 54.0 : 1.0 / 216.0;
+		const real cu = (real) CX[q] * vx + (real) CY[q] * vy + (real) CZ[q] * vz;
+		const real feq = (real) 1 - (real) 1.5 * (vx * vx + vy * vy + vz * vz) + (real) 3 * cu + (real) 4.5 * cu * cu;
+		return (real) w * rho * feq;
+	}
+	template <int KIND, int q>
+	static LBM_DEV real eq(real rho, real vx, real vy, real vz)
+	{
+		if constexpr (KIND == 0)
+			return eq_invcum<q>(rho, vx, vy, vz);
+		else
+			return eq_poly<q>(rho, vx, vy, vz);
+	}
+
+	// forward 1-D central-moment pass (col_cum.h Eqs 6-14)
+	static LBM_DEV void fwd(real fm, real fz, real fp, real v, real v2, real& k0, real& k1, real& k2)
+	{
+		const real s = fp + fm, d = fp - fm;
+		k0 = s + fz;
+		k1 = d - v * k0;
+		k2 = s - (real) 2 * v * d + v2 * k0;
+	}
+	// backward 1-D pass (col_cum.h Eqs 88-96)
+	static LBM_DEV void bwd(real k0, real k1, real k2, real v, real v2, real& fm, real& fz, real& fp)
+	{
+		fz = k0 * ((real) 1 - v2) - (real) 2 * v * k1 - k2;
+		fm = (k0 * (v2 - v) + k1 * ((real) 2 * v - (real) 1) + k2) * (real) 0.5;
+		fp = (k0 * (v2 + v) + k1 * ((real) 2 * v + (real) 1) + k2) * (real) 0.5;
+	}
+
+	// D3Q27_CUM::collision (col_cum.h:13-511).  omega1 = 1/(3 nu + 1/2) is
+	// precomputed on the host exactly as the reference computes it.
+	template <bool G2017, bool AA>
+	static LBM_DEV void cumulant(real* f, real rho, real vx, real vy, real vz, real omega1)
+	{
+		const real rho_inv = (real) 1 / rho;
+		const real vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+		real T[3][3][3], U[3][3][3], k[3][3][3];
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int b = 0; b < 3; b++)
+				fwd(f[qidx(a, b, 0)], f[qidx(a, b, 1)], f[qidx(a, b, 2)], vz, vz2, T[a][b][0], T[a][b][1], T[a][b][2]);
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				fwd(T[a][0][j], T[a][1][j], T[a][2][j], vy, vy2, U[a][0][j], U[a][1][j], U[a][2][j]);
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				fwd(U[0][i][j], U[1][i][j], U[2][i][j], vx, vx2, k[0][i][j], k[1][i][j], k[2][i][j]);
+
+		const real one = 1, two = 2, three = 3, four = 4, half = (real) 0.5, third = (real) (1.0 / 3.0);
+		const real omega2 = one;
+		const real k_000 = k[0][0][0];
+		const real C_200 = k[2][0][0], C_020 = k[0][2][0], C_002 = k[0][0][2];
+		const real C_110 = k[1][1][0], C_101 = k[1][0][1], C_011 = k[0][1][1];
+		real Ks[3][3][3];
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int b = 0; b < 3; b++)
+#pragma unroll
+				for (int c = 0; c < 3; c++)
+					Ks[a][b][c] = 0;
+		const real ks_110 = (one - omega1) * C_110;
+		const real ks_101 = (one - omega1) * C_101;
+		const real ks_011 = (one - omega1) * C_011;
+		if constexpr (! G2017) {
+			// Eqs 33-35 without antialiasing (col_cum.h:247-259)
+			const real Eq33 = (one - omega1) * (C_200 - C_020);
+			const real Eq34 = (one - omega1) * (C_200 - C_002);
+			const real Eq35 = k_000 * omega2 + (one - omega2) * (C_200 + C_020 + C_002);
+			const real ks_200 = third * (Eq33 + Eq34 + Eq35);
+			const real ks_020 = third * (-two * Eq33 + Eq34 + Eq35);
+			const real ks_002 = third * (Eq33 - two * Eq34 + Eq35);
+			// col_cum.h:346-365
+			const real ks_211 = (ks_200 * ks_011 + two * ks_101 * ks_110) * rho_inv;
+			const real ks_121 = (ks_020 * ks_101 + two * ks_110 * ks_011) * rho_inv;
+			const real ks_112 = (ks_002 * ks_110 + two * ks_011 * ks_101) * rho_inv;
+			const real ks_220 = (ks_020 * ks_200 + two * ks_110 * ks_110) * rho_inv;
+			const real ks_022 = (ks_002 * ks_020 + two * ks_011 * ks_011) * rho_inv;
+			const real ks_202 = (ks_200 * ks_002 + two * ks_101 * ks_101) * rho_inv;
+			const real ks_222 =
+				(ks_200 * ks_022 + ks_020 * ks_202 + ks_002 * ks_220 + four * (ks_011 * ks_211 + ks_101 * ks_121 + ks_110 * ks_112)) * rho_inv
+				- ((real) 16 * ks_110 * ks_101 * ks_011 + four * (ks_101 * ks_101 * ks_020 + ks_011 * ks_011 * ks_200 + ks_110 * ks_110 * ks_002)
+				   + two * ks_200 * ks_020 * ks_002)
+					  * rho_inv * rho_inv;
+			Ks[2][0][0] = ks_200; Ks[0][2][0] = ks_020; Ks[0][0][2] = ks_002;
+			Ks[2][1][1] = ks_211; Ks[1][2][1] = ks_121; Ks[1][1][2] = ks_112;
+			Ks[2][2][0] = ks_220; Ks[0][2][2] = ks_022; Ks[2][0][2] = ks_202;
+			Ks[2][2][2] = ks_222;
+		}
+		else {
+			const real k_111 = k[1][1][1];
+			const real k_120 = k[1][2][0], k_102 = k[1][0][2], k_210 = k[2][1][0], k_012 = k[0][1][2], k_201 = k[2][0][1], k_021 = k[0][2][1];
+			const real k_211 = k[2][1][1], k_121 = k[1][2][1], k_112 = k[1][1][2];
+			const real k_220 = k[2][2][0], k_202 = k[2][0][2], k_022 = k[0][2][2];
+			const real k_122 = k[1][2][2], k_212 = k[2][1][2], k_221 = k[2][2][1], k_222 = k[2][2][2];
+			const real k_200 = C_200, k_020 = C_020, k_002 = C_002, k_110 = C_110, k_101 = C_101, k_011 = C_011;
+			// cumulants of order 4-6 (Eqs 51-54 of Geier 2015)
+			const real C_211 = k_211 - (k_200 * k_011 + two * k_101 * k_110) * rho_inv;
+			const real C_121 = k_121 - (k_020 * k_101 + two * k_110 * k_011) * rho_inv;
+			const real C_112 = k_112 - (k_002 * k_110 + two * k_011 * k_101) * rho_inv;
+			const real C_220 = k_220 - (k_020 * k_200 + two * k_110 * k_110) * rho_inv;
+			const real C_022 = k_022 - (k_002 * k_020 + two * k_011 * k_011) * rho_inv;
+			const real C_202 = k_202 - (k_200 * k_002 + two * k_101 * k_101) * rho_inv;
+			const real C_122 = k_122 - (k_020 * k_102 + k_002 * k_120 + four * k_011 * k_111 + two * (k_110 * k_012 + k_101 * k_021)) * rho_inv;
+			const real C_212 = k_212 - (k_002 * k_210 + k_200 * k_012 + four * k_101 * k_111 + two * (k_011 * k_201 + k_110 * k_102)) * rho_inv;
+			const real C_221 = k_221 - (k_200 * k_021 + k_020 * k_201 + four * k_110 * k_111 + two * (k_101 * k_120 + k_011 * k_210)) * rho_inv;
+			const real C_222 = k_222
+							 - (four * k_111 * k_111 + k_200 * k_022 + k_020 * k_202 + k_002 * k_220
+								+ four * (k_011 * k_211 + k_101 * k_121 + k_110 * k_112) + two * (k_120 * k_102 + k_210 * k_012 + k_201 * k_021))
+								   * rho_inv
+							 + ((real) 16 * k_110 * k_101 * k_011 + four * (k_101 * k_101 * k_020 + k_011 * k_011 * k_200 + k_110 * k_110 * k_002)
+								+ two * k_200 * k_020 * k_002)
+								   * rho_inv * rho_inv;
+			// relaxation rates and limiters of Geier 2017 (col_cum.h:198-227)
+			const real o1 = omega1, o2 = omega2;
+			const real lambda = (real) 0.01;
+			const real omega3 = (real) 8 * (o1 - two) * (o2 * (three * o1 - one) - (real) 5 * o1)
+							  / ((real) 8 * ((real) 5 - two * o1) * o1 + o2 * ((real) 8 + o1 * ((real) 9 * o1 - (real) 26)));
+			const real omega4 = (real) 8 * (o1 - two) * (o1 + o2 * (three * o1 - (real) 7))
+							  / (o2 * ((real) 56 - (real) 42 * o1 + (real) 9 * o1 * o1) - (real) 8 * o1);
+			const real omega5 = (real) 24 * (o1 - two)
+							  * (four * o1 * o1 + o1 * o2 * ((real) 18 - (real) 13 * o1) + o2 * o2 * (two + o1 * ((real) 6 * o1 - (real) 11)))
+							  / ((real) 16 * o1 * o1 * (o1 - (real) 6) - two * o1 * o2 * ((real) 216 + (real) 5 * o1 * ((real) 9 * o1 - (real) 46))
+								 + o2 * o2 * (o1 * (three * o1 - (real) 10) * ((real) 15 * o1 - (real) 28) - (real) 48));
+			auto lim = [&](real om, real c) { return om + (one - om) * fabs(c) / (rho * lambda + fabs(c)); };
+			const real A = (four * o1 * o1 + two * o1 * o2 * (o1 - (real) 6) + o2 * o2 * (o1 * ((real) 10 - three * o1) - four))
+						 / (o1 - o2) / (o2 * (two + three * o1) - (real) 8 * o1);
+			const real B = (four * o1 * o2 * ((real) 9 * o1 - (real) 16) - four * o1 * o1 - two * o2 * o2 * (two + (real) 9 * o1 * (o1 - two)))
+						 * third / (o1 - o2) / (o2 * (two + three * o1) - (real) 8 * o1);
+			real Dxu = 0, Dyv = 0, Dzw = 0, DxvDyu = 0, DxwDzu = 0, DywDzv = 0;
+			if constexpr (AA) {
+				// col_cum.h:237-245, trace term as written (subtracts rho - 1)
+				Dxu = -o1 * half * rho_inv * (two * C_200 - C_020 - C_002) - o2 * half * rho_inv * (C_200 + C_020 + C_002 - (-one + rho));
+				Dyv = Dxu + (real) 1.5 * o1 * rho_inv * (C_200 - C_020);
+				Dzw = Dxu + (real) 1.5 * o1 * rho_inv * (C_200 - C_002);
+				DxvDyu = -three * o1 * rho_inv * C_110;
+				DxwDzu = -three * o1 * rho_inv * C_101;
+				DywDzv = -three * o1 * rho_inv * C_011;
+			}
+			const real Eq33 = (one - o1) * (C_200 - C_020) - three * rho * (one - o1 * half) * (vx2 * Dxu - vy2 * Dyv);
+			const real Eq34 = (one - o1) * (C_200 - C_002) - three * rho * (one - o1 * half) * (vx2 * Dxu - vz2 * Dzw);
+			const real Eq35 = k_000 * o2 + (one - o2) * (C_200 + C_020 + C_002) - three * rho * (one - o2 * half) * (vx2 * Dxu + vy2 * Dyv + vz2 * Dzw);
+			const real ks_200 = third * (Eq33 + Eq34 + Eq35);
+			const real ks_020 = third * (-two * Eq33 + Eq34 + Eq35);
+			const real ks_002 = third * (Eq33 - two * Eq34 + Eq35);
+			const real Eq117 = (one - lim(omega3, k_120 + k_102)) * (k_120 + k_102);
+			const real Eq118 = (one - lim(omega3, k_210 + k_012)) * (k_210 + k_012);
+			const real Eq119 = (one - lim(omega3, k_201 + k_021)) * (k_201 + k_021);
+			const real Eq120 = (one - lim(omega4, k_120 - k_102)) * (k_120 - k_102);
+			const real Eq121 = (one - lim(omega4, k_210 - k_012)) * (k_210 - k_012);
+			const real Eq122 = (one - lim(omega4, k_201 - k_021)) * (k_201 - k_021);
+			const real ks_120 = half * (Eq120 + Eq117);
+			const real ks_102 = half * (-Eq120 + Eq117);
+			const real ks_210 = half * (Eq121 + Eq118);
+			const real ks_012 = half * (-Eq121 + Eq118);
+			const real ks_021 = half * (-Eq122 + Eq119);
+			const real ks_201 = half * (Eq122 + Eq119);
+			const real ks_111 = (one - lim(omega5, k_111)) * k_111;
+			const real visc = (one / o1 - half);
+			const real Eq43 = (real) (2.0 / 3.0) * visc * A * rho * (Dxu - two * Dyv + Dzw);
+			const real Eq44 = (real) (2.0 / 3.0) * visc * A * rho * (Dxu + Dyv - two * Dzw);
+			const real Eq45 = -(real) (4.0 / 3.0) * visc * A * rho * (Dxu + Dyv + Dzw);
+			const real Cs_220 = third * (Eq43 + Eq44 + Eq45);
+			const real Cs_202 = third * (-Eq43 + Eq45);
+			const real Cs_022 = third * (-Eq44 + Eq45);
+			const real Cs_211 = -third * visc * B * rho * DywDzv;
+			const real Cs_121 = -third * visc * B * rho * DxwDzu;
+			const real Cs_112 = -third * visc * B * rho * DxvDyu;
+			(void) C_211; (void) C_121; (void) C_112; (void) C_220; (void) C_022; (void) C_202;
+			(void) C_122; (void) C_212; (void) C_221; (void) C_222;
+			// backward cumulant transform (Eqs 81-84); omega6..10 = 1 zero out the rest
+			const real ks_211 = Cs_211 + (ks_200 * ks_011 + two * ks_101 * ks_110) * rho_inv;
+			const real ks_121 = Cs_121 + (ks_020 * ks_101 + two * ks_110 * ks_011) * rho_inv;
+			const real ks_112 = Cs_112 + (ks_002 * ks_110 + two * ks_011 * ks_101) * rho_inv;
+			const real ks_220 = Cs_220 + (ks_020 * ks_200 + two * ks_110 * ks_110) * rho_inv;
+			const real ks_022 = Cs_022 + (ks_002 * ks_020 + two * ks_011 * ks_011) * rho_inv;
+			const real ks_202 = Cs_202 + (ks_200 * ks_002 + two * ks_101 * ks_101) * rho_inv;
+			const real ks_122 = (ks_020 * ks_102 + ks_002 * ks_120 + four * ks_011 * ks_111 + two * (ks_110 * ks_012 + ks_101 * ks_021)) * rho_inv;
+			const real ks_212 = (ks_002 * ks_210 + ks_200 * ks_012 + four * ks_101 * ks_111 + two * (ks_011 * ks_201 + ks_110 * ks_102)) * rho_inv;
+			const real ks_221 = (ks_200 * ks_021 + ks_020 * ks_201 + four * ks_110 * ks_111 + two * (ks_101 * ks_120 + ks_011 * ks_210)) * rho_inv;
+			const real ks_222 =
+				(four * ks_111 * ks_111 + ks_200 * ks_022 + ks_020 * ks_202 + ks_002 * ks_220
+				 + four * (ks_011 * ks_211 + ks_101 * ks_121 + ks_110 * ks_112) + two * (ks_120 * ks_102 + ks_210 * ks_012 + ks_201 * ks_021))
+					* rho_inv
+				- ((real) 16 * ks_110 * ks_101 * ks_011 + four * (ks_101 * ks_101 * ks_020 + ks_011 * ks_011 * ks_200 + ks_110 * ks_110 * ks_002)
+				   + two * ks_200 * ks_020 * ks_002)
+					  * rho_inv * rho_inv;
+			Ks[2][0][0] = ks_200; Ks[0][2][0] = ks_020; Ks[0][0][2] = ks_002;
+			Ks[1][1][1] = ks_111; Ks[1][2][0] = ks_120; Ks[1][0][2] = ks_102; Ks[2][1][0] = ks_210;
+			Ks[0][1][2] = ks_012; Ks[2][0][1] = ks_201; Ks[0][2][1] = ks_021;
+			Ks[2][1][1] = ks_211; Ks[1][2][1] = ks_121; Ks[1][1][2] = ks_112;
+			Ks[2][2][0] = ks_220; Ks[0][2][2] = ks_022; Ks[2][0][2] = ks_202;
+			Ks[1][2][2] = ks_122; Ks[2][1][2] = ks_212; Ks[2][2][1] = ks_221; Ks[2][2][2] = ks_222;
+		}
+		Ks[1][1][0] = ks_110; Ks[1][0][1] = ks_101; Ks[0][1][1] = ks_011;
+		Ks[0][0][0] = k_000;
+		Ks[1][0][0] = -k[1][0][0];  // Geier 2017 forcing scheme (col_cum.h:371-373)
+		Ks[0][1][0] = -k[0][1][0];
+		Ks[0][0][1] = -k[0][0][1];
+#pragma unroll
+		for (int j = 0; j < 3; j++)
+#pragma unroll
+			for (int c = 0; c < 3; c++)
+				bwd(Ks[0][j][c], Ks[1][j][c], Ks[2][j][c], vx, vx2, U[0][j][c], U[1][j][c], U[2][j][c]);
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int c = 0; c < 3; c++)
+				bwd(U[a][0][c], U[a][1][c], U[a][2][c], vy, vy2, T[a][0][c], T[a][1][c], T[a][2][c]);
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int b = 0; b < 3; b++)
+				bwd(T[a][b][0], T[a][b][1], T[a][b][2], vz, vz2, f[qidx(a, b, 0)], f[qidx(a, b, 1)], f[qidx(a, b, 2)]);
+	}
+};
+
+// ---------------------------------------------------------------------------
+// Deviation form (g = f - w, float arithmetic): every chain value is carried
+// as its deviation from the value it takes at rest (rho = 1, u = 0); rest
+// values are products of the 1-D weight moments (1, 0, 1/3).  Same algebra
+// as the reference's well-conditioned cumulant (col_cum_well.h) restricted
+// to the default relaxation.
+struct Dev
+{
+	using real = float;
+
+	static LBM_DEV void density_velocity(const real* f, real Fx, real Fy, real Fz, real& rho, real& vx, real& vy, real& vz)
+	{
+		rho = ((((f[ppp] + f[mmm]) + (f[pmp] + f[mpm])) + ((f[ppm] + f[mmp]) + (f[mpp] + f[pmm])))
+			   + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[pzp] + f[mzm]) + (f[pzm] + f[mzp]))
+				  + ((f[ppz] + f[mmz]) + (f[pmz] + f[mpz])))
+			   + ((f[pzz] + f[mzz]) + (f[zpz] + f[zmz]) + (f[zzp] + f[zzm])))
+			+ f[zzz];
+		rho += 1.0f;
+		const real rinv = 1.0f / rho;
+		vx = ((((f[ppp] - f[mmm]) + (f[pmp] - f[mpm])) + ((f[ppm] - f[mmp]) + (f[pmm] - f[mpp])))
+			  + (((f[pzp] - f[mzm]) + (f[pzm] - f[mzp])) + ((f[ppz] - f[mmz]) + (f[pmz] - f[mpz]))) + (f[pzz] - f[mzz])
+			  + Fx * 0.5f)
+			* rinv;
+		vy = ((((f[ppp] - f[mmm]) + (f[ppm] - f[mmp])) + ((f[mpp] - f[pmm]) + (f[mpm] - f[pmp])))
+			  + (((f[ppz] - f[mmz]) + (f[mpz] - f[pmz])) + ((f[zpp] - f[zmm]) + (f[zpm] - f[zmp]))) + (f[zpz] - f[zmz])
+			  + Fy * 0.5f)
+			* rinv;
+		vz = ((((f[ppp] - f[mmm]) + (f[mpp] - f[pmm])) + ((f[pmp] - f[mpm]) + (f[mmp] - f[ppm])))
+			  + (((f[zpp] - f[zmm]) + (f[zmp] - f[zpm])) + ((f[pzp] - f[mzm]) + (f[mzp] - f[pzm]))) + (f[zzp] - f[zzm])
+			  + Fz * 0.5f)
+			* rinv;
+	}
+
+	template <int c>
+	static LBM_DEV real dW(real v)
+	{
+		if constexpr (c == 0)
+			return -v * v;
+		else if constexpr (c > 0)
+			return 0.5f * (v * v + v);
+		else
+			return 0.5f * (v * v - v);
+	}
+	// product-form equilibrium as a deviation: rho*prod(W0+dW) - prod(W0)
+	template <int q>
+	static LBM_DEV real eq_invcum(real rho, real vx, real vy, real vz)
+	{
+		constexpr float wx = (float) W1[CX[q] + 1], wy = (float) W1[CY[q] + 1], wz = (float) W1[CZ[q] + 1];
+		const real dx = dW<CX[q]>(vx), dy = dW<CY[q]>(vy), dz = dW<CZ[q]>(vz);
+		const real d2 = dx * (wy + dy) + wx * dy;
+		const real d3 = d2 * (wz + dz) + (wx * wy) * dz;
+		return d3 + (rho - 1.0f) * ((wx * wy * wz) + d3);
+	}
+	template <int q>
+	static LBM_DEV real eq_poly(real rho, real vx, real vy, real vz)
+	{
+		constexpr int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+		constexpr float w = nz == 0 ? (float) (8.0 / 27.0) : nz == 1 ? (float) (2.0 / 27.0) : nz == 2 ? (float) (1.0 / 54.0) : (float) (1.0 / 216.0);
+		const real cu = (real) CX[q] * vx + (real) CY[q] * vy + (real) CZ[q] * vz;
+		const real d = -1.5f * (vx * vx + vy * vy + vz * vz) + 3.0f * cu + 4.5f * cu * cu;
+		return w * (d + (rho - 1.0f) * (1.0f + d));
+	}
+	template <int KIND, int q>
+	static LBM_DEV real eq(real rho, real vx, real vy, real vz)
+	{
+		if constexpr (KIND == 0)
+			return eq_invcum<q>(rho, vx, vy, vz);
+		else
+			return eq_poly<q>(rho, vx, vy, vz);
+	}
+
+	static LBM_DEV void fwd(real dm, real dz, real dp, real R0, real v, real v2, real& k0, real& k1, real& k2)
+	{
+		const real s = dp + dm, d = dp - dm;
+		k0 = s + dz;
+		const real full = R0 + k0;
+		k1 = d - v * full;
+		k2 = s - 2.0f * v * d + v2 * full;
+	}
+	static LBM_DEV void bwd(real k0, real k1, real k2, real R0, real v, real v2, real& dm, real& dz, real& dp)
+	{
+		const real full = R0 + k0;
+		dz = k0 * (1.0f - v2) - 2.0f * v * k1 - k2 - R0 * v2;
+		dm = (full * (v2 - v) + k1 * (2.0f * v - 1.0f) + k2) * 0.5f;
+		dp = (full * (v2 + v) + k1 * (2.0f * v + 1.0f) + k2) * 0.5f;
+	}
+	static constexpr float M(int i)
+	{
+		return i == 0 ? 1.0f : i == 1 ? 0.0f : (float) (1.0 / 3.0);
+	}
+	static constexpr float Wf(int a)
+	{
+		return (float) W1[a];
+	}
+
+	static LBM_DEV void cumulant(real* g, real rho, real vx, real vy, real vz, real omega1)
+	{
+		const real drho = rho - 1.0f;
+		const real rho_inv = 1.0f / rho;
+		const real vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+		real T[3][3][3], U[3][3][3], k[3][3][3];
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int b = 0; b < 3; b++)
+				fwd(g[qidx(a, b, 0)], g[qidx(a, b, 1)], g[qidx(a, b, 2)], Wf(a) * Wf(b), vz, vz2, T[a][b][0], T[a][b][1], T[a][b][2]);
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				fwd(T[a][0][j], T[a][1][j], T[a][2][j], Wf(a) * M(j), vy, vy2, U[a][0][j], U[a][1][j], U[a][2][j]);
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				fwd(U[0][i][j], U[1][i][j], U[2][i][j], M(i) * M(j), vx, vx2, k[0][i][j], k[1][i][j], k[2][i][j]);
+		const real third = (float) (1.0 / 3.0), ninth = (float) (1.0 / 9.0);
+		const real one_m = 1.0f - omega1;
+		const real C_200 = k[2][0][0], C_020 = k[0][2][0], C_002 = k[0][0][2];
+		const real Eq33 = one_m * (C_200 - C_020);
+		const real Eq34 = one_m * (C_200 - C_002);
+		const real Eq35 = drho;  // omega2 = 1
+		const real a = third * (Eq33 + Eq34 + Eq35);
+		const real b = third * (-2.0f * Eq33 + Eq34 + Eq35);
+		const real c = third * (Eq33 - 2.0f * Eq34 + Eq35);
+		const real s = one_m * k[1][1][0], p = one_m * k[1][0][1], qq = one_m * k[0][1][1];
+		const real ks_211 = ((third + a) * qq + 2.0f * p * s) * rho_inv;
+		const real ks_121 = ((third + b) * p + 2.0f * s * qq) * rho_inv;
+		const real ks_112 = ((third + c) * s + 2.0f * qq * p) * rho_inv;
+		const real ks_220 = ((b * a + 2.0f * s * s) + (b + a) * third - drho * ninth) * rho_inv;
+		const real ks_022 = ((c * b + 2.0f * qq * qq) + (c + b) * third - drho * ninth) * rho_inv;
+		const real ks_202 = ((a * c + 2.0f * p * p) + (a + c) * third - drho * ninth) * rho_inv;
+		const real X1 = (a + b + c) * ninth + (ks_022 + ks_202 + ks_220) * third + a * ks_022 + b * ks_202 + c * ks_220
+					  + 4.0f * (qq * ks_211 + p * ks_121 + s * ks_112);
+		const real X2 = (float) (2.0 / 3.0) * ((a + b + c) * third + (a * b + b * c + c * a) + 2.0f * (p * p + qq * qq + s * s))
+					  + 16.0f * s * p * qq + 4.0f * (p * p * b + qq * qq * a + s * s * c) + 2.0f * a * b * c;
+		const real ks_222 = (X1 - (X2 + drho * (drho - 1.0f) * (float) (1.0 / 27.0)) * rho_inv) * rho_inv;
+		real Ks[3][3][3];
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+#pragma unroll
+				for (int l = 0; l < 3; l++)
+					Ks[i][j][l] = 0.0f;
+		Ks[0][0][0] = drho;
+		Ks[1][0][0] = -k[1][0][0];
+		Ks[0][1][0] = -k[0][1][0];
+		Ks[0][0][1] = -k[0][0][1];
+		Ks[2][0][0] = a; Ks[0][2][0] = b; Ks[0][0][2] = c;
+		Ks[1][1][0] = s; Ks[1][0][1] = p; Ks[0][1][1] = qq;
+		Ks[2][1][1] = ks_211; Ks[1][2][1] = ks_121; Ks[1][1][2] = ks_112;
+		Ks[2][2][0] = ks_220; Ks[0][2][2] = ks_022; Ks[2][0][2] = ks_202;
+		Ks[2][2][2] = ks_222;
+#pragma unroll
+		for (int j = 0; j < 3; j++)
+#pragma unroll
+			for (int l = 0; l < 3; l++)
+				bwd(Ks[0][j][l], Ks[1][j][l], Ks[2][j][l], M(j) * M(l), vx, vx2, U[0][j][l], U[1][j][l], U[2][j][l]);
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int l = 0; l < 3; l++)
+				bwd(U[i][0][l], U[i][1][l], U[i][2][l], Wf(i) * M(l), vy, vy2, T[i][0][l], T[i][1][l], T[i][2][l]);
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				bwd(T[i][j][0], T[i][j][1], T[i][j][2], Wf(i) * Wf(j), vz, vz2, g[qidx(i, j, 0)], g[qidx(i, j, 1)], g[qidx(i, j, 2)]);
+	}
+};
+
+template <int ST>
+using Phys = typename std::conditional<Storage<ST>::dev, Dev, Abs<typename Storage<ST>::real>>::type;
+
+}  // namespace lbm

@@ -1,0 +1,15 @@
+"""pytnl_lbm: the MI355X (gfx950) lattice update of TNL-LBM, exposed to PyTorch.
+
+Mirrors the reference's Python module surface for the lattice update
+(TNL-LBM pytnl_lbm/pytnl_lbm.cpp:15-73): a lattice block whose step is the
+fused collide+stream+BC kernel, initial conditions, macroscopic output and
+FP64 reductions.  Every operation runs in libtnl_lbm_amd.so (HIP, gfx950);
+nothing falls back to the CPU.
+"""
+from . import _abi
+from ._abi import ADE_GEO, GEO, LBMError
+from .lattice import D3Q27Lattice, from_storage, to_storage
+from .fields import tgv_fields, turbulence_fields
+
+__all__ = ["_abi", "GEO", "ADE_GEO", "LBMError", "D3Q27Lattice", "to_storage", "from_storage",
+           "tgv_fields", "turbulence_fields"]

@@ -1,0 +1,123 @@
+"""Initial-condition fields (host, FP64 numpy), uploaded by the caller.
+
+Generated on the host in double precision so that every backend starts from
+bit-identical (rho, u) arrays.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def tgv_coordinates(N: int) -> np.ndarray:
+    """Cell-centred coordinates of sim_4's lattice (sim_4.cu:340-348,
+    lattice.h:71-74): x_i = origin + (i - 1/2) dl with dl = 2 pi / N and
+    origin = -dl N / 2 + dl, i.e. x_i = dl (i + 1/2) - pi."""
+    dl = 2 * np.pi / N
+    origin = -dl / 2 * N + dl
+    return origin + (np.arange(N, dtype=np.float64) - 0.5) * dl
+
+
+def tgv_fields(X: int, Y: int, Z: int, U0: float):
+    """Taylor-Green vortex of sim_4.cu:145-163 with L = 1 and V_0 = U0 in
+    lattice units: u = U0 sin x cos y cos z, v = -U0 cos x sin y cos z, w = 0,
+    rho = 1 + 3 (U0^2/16)(cos 2x + cos 2y)(cos 2z + 2).  Returns [x, y, z]
+    arrays (rho, vx, vy, vz)."""
+    x = tgv_coordinates(X)[:, None, None]
+    y = tgv_coordinates(Y)[None, :, None]
+    z = tgv_coordinates(Z)[None, None, :]
+    u = U0 * np.sin(x) * np.cos(y) * np.cos(z)
+    v = -U0 * np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros((X, Y, Z))
+    rho = 1 + 3 * (U0 * U0 / 16) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2)
+    shape = (X, Y, Z)
+    return (np.broadcast_to(rho, shape).copy(), np.broadcast_to(u, shape).copy(),
+            np.broadcast_to(v, shape).copy(), w)
+
+
+def turbulence_fields(N: int, seed: int, k0: float, u_rms: float, chunk_z: int | None = None):
+    """Seeded solenoidal random velocity on an N^3 periodic box with energy
+    spectrum E(k) ~ k^4 exp(-2 (k/k0)^2) (integer box wavenumbers), zero mean,
+    Nyquist modes removed, scaled to u_rms = sqrt(<|u|^2>/3); rho = 1.
+    Returns float64 [x, y, z] arrays (rho, vx, vy, vz)."""
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(N, d=1.0 / N)
+    kz = np.fft.rfftfreq(N, d=1.0 / N)
+    KX, KY, KZ = np.meshgrid(k, k, kz, indexing="ij")
+    K2 = KX * KX + KY * KY + KZ * KZ
+    Kmag = np.sqrt(K2)
+    amp = np.zeros_like(Kmag)
+    nz = Kmag > 0
+    E = Kmag[nz] ** 4 * np.exp(-2.0 * (Kmag[nz] / k0) ** 2)
+    amp[nz] = np.sqrt(E / (4 * np.pi * K2[nz]))
+    nyq = (np.abs(KX) == N // 2) | (np.abs(KY) == N // 2) | (np.abs(KZ) == N // 2)
+    amp[nyq] = 0.0
+    uh = []
+    for _ in range(3):
+        re = rng.standard_normal(Kmag.shape)
+        im = rng.standard_normal(Kmag.shape)
+        uh.append((re + 1j * im) * amp)
+    del amp, re, im
+    # solenoidal projection u <- u - k (k.u)/|k|^2
+    K2s = np.where(nz, K2, 1.0)
+    div = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / K2s
+    uh[0] -= KX * div
+    uh[1] -= KY * div
+    uh[2] -= KZ * div
+    del div
+    u = [np.fft.irfftn(h, s=(N, N, N), axes=(0, 1, 2)) for h in uh]
+    del uh
+    for c in u:
+        c -= c.mean()
+    rms = np.sqrt((u[0] ** 2 + u[1] ** 2 + u[2] ** 2).mean() / 3.0)
+    scale = u_rms / rms
+    for c in u:
+        c *= scale
+    rho = np.ones((N, N, N))
+    return rho, u[0], u[1], u[2]
+
+
+def turbulence_fields_torch(N: int, seed: int, k0: float, u_rms: float, device="cuda", dtype=None):
+    """turbulence_fields() on a torch device (same spectrum, projection and
+    normalisation; the random stream is torch's).  Returns [x, y, z] float64
+    tensors (vx, vy, vz) on `device`; rho = 1 is implied."""
+    import torch
+
+    dt = dtype or torch.float64
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    k = torch.fft.fftfreq(N, d=1.0 / N, device=device, dtype=dt)
+    kz = torch.fft.rfftfreq(N, d=1.0 / N, device=device, dtype=dt)
+    KX, KY, KZ = k[:, None, None], k[None, :, None], kz[None, None, :]
+    K2 = KX * KX + KY * KY + KZ * KZ
+    Kmag = torch.sqrt(K2)
+    E = Kmag ** 4 * torch.exp(-2.0 * (Kmag / k0) ** 2)
+    amp = torch.sqrt(E / (4 * torch.pi * torch.where(K2 > 0, K2, torch.ones_like(K2))))
+    amp[0, 0, 0] = 0
+    half = N // 2
+    amp[half, :, :] = 0
+    amp[:, half, :] = 0
+    amp[:, :, -1] = 0 if N % 2 == 0 else amp[:, :, -1]
+    del E, Kmag
+    shape = amp.shape
+    uh = []
+    for _ in range(3):
+        re = torch.randn(shape, generator=g, device=device, dtype=dt)
+        im = torch.randn(shape, generator=g, device=device, dtype=dt)
+        uh.append(torch.complex(re * amp, im * amp))
+        del re, im
+    del amp
+    K2s = torch.where(K2 > 0, K2, torch.ones_like(K2))
+    div = (KX * uh[0] + KY * uh[1] + KZ * uh[2]) / K2s
+    uh[0] -= KX * div
+    uh[1] -= KY * div
+    uh[2] -= KZ * div
+    del div, K2s, K2
+    u = []
+    for h in uh:
+        u.append(torch.fft.irfftn(h, s=(N, N, N), dim=(0, 1, 2)))
+    del uh
+    for c in u:
+        c -= c.mean()
+    rms = torch.sqrt((u[0] ** 2 + u[1] ** 2 + u[2] ** 2).mean() / 3.0)
+    s = u_rms / rms
+    return [c * s for c in u]

@@ -1,0 +1,192 @@
+"""Device lattice objects driving libtnl_lbm_amd.so through its C ABI.
+
+PyTorch provides device memory and the stream; every lattice operation is a
+call into the HIP library.  There is no CPU fallback: constructing a lattice
+without the built library, or without a GPU, raises.
+
+Storage layout is the reference's (TNL-LBM defs.h:90, lbm_data.h:50-54):
+populations as a (27, X, Z, Y) tensor, i.e. index q*XYZ + (x*Z + z)*Y + y.
+The `*_xyz` accessors return [x, y, z]-ordered views of it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import torch
+
+from . import _abi
+from ._abi import GEO
+
+_TORCH_ELEM = {_abi.STORE_F64: torch.float64, _abi.STORE_F32: torch.float32, _abi.STORE_F32_DEV: torch.float32,
+               _abi.STORE_F16_DEV: torch.float16, _abi.STORE_BF16_DEV: torch.bfloat16}
+
+
+def _stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def to_storage(a: torch.Tensor) -> torch.Tensor:
+    """[x, y, z] array -> contiguous storage order (x, z, y)."""
+    return a.permute(0, 2, 1).contiguous()
+
+
+def from_storage(a: torch.Tensor) -> torch.Tensor:
+    """storage (x, z, y) -> [x, y, z] view."""
+    return a.permute(0, 2, 1)
+
+
+class D3Q27Lattice:
+    """One lattice block on one GPU: the device side of LBM_BLOCK
+    (lbm_block.hpp:549-617) plus the step driver of State::SimUpdate
+    (state.hpp:1235-1409) and LBM::updateKernelData (lbm.hpp:307-323)."""
+
+    def __init__(self, X: int, Y: int, Z: int, *, collision="CUM", storage="f64", pattern="AA",
+                 macro_real=None, nu: float = 0.0, force=(0.0, 0.0, 0.0), geo=None, device="cuda:0",
+                 XYZ: int | None = None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("D3Q27Lattice needs a GPU (libtnl_lbm_amd.so runs on gfx950 only)")
+        self.lib = _abi.load_library()
+        coll = _abi.COLLISION_NAMES[collision] if isinstance(collision, str) else collision
+        st = _abi.STORAGE_NAMES[storage] if isinstance(storage, str) else storage
+        pat = {"AA": _abi.PATTERN_AA, "AB": _abi.PATTERN_AB}[pattern] if isinstance(pattern, str) else pattern
+        if macro_real is None:
+            macro_real = _abi.REAL_F64 if st == _abi.STORE_F64 else _abi.REAL_F32
+        self.cfg = _abi.Config.make(coll, st, pat, macro_real)
+        self.d = _abi.D3Q27Data.make(X, Y, Z, XYZ=XYZ)
+        self.X, self.Y, self.Z = X, Y, Z
+        self.device = torch.device(device)
+        n = self.d.XYZ
+        el = _TORCH_ELEM[st]
+        ncop = 2 if pat == _abi.PATTERN_AB else 1
+        self.df = [torch.empty(27 * n, dtype=el, device=self.device) for _ in range(ncop)]
+        mdt = torch.float64 if macro_real == _abi.REAL_F64 else torch.float32
+        self.macro = torch.zeros(4 * n, dtype=mdt, device=self.device)
+        self.map = None
+        self.d.lbm_viscosity = nu
+        self.d.fx, self.d.fy, self.d.fz = force
+        self._profiles = [None, None, None]
+        if geo is not None:
+            self.set_geometry(geo)
+        self.iterations = 0
+        self._bind()
+
+    # ---- bookkeeping --------------------------------------------------------
+    @property
+    def storage(self):
+        return self.cfg.storage
+
+    @property
+    def nsites(self):
+        return self.X * self.Y * self.Z
+
+    def _bind(self):
+        self.d.df[0] = self.df[0].data_ptr()
+        self.d.df[1] = self.df[1].data_ptr() if len(self.df) > 1 else None
+        self.d.macro = self.macro.data_ptr()
+        self.d.map = self.map.data_ptr() if self.map is not None else None
+
+    def _check(self, rc):
+        _abi.check(self.lib, rc)
+
+    def set_geometry(self, geo):
+        """geo: int16 [x, y, z] array of GEO codes (None = all periodic)."""
+        if geo is None:
+            self.map = None
+        else:
+            g = torch.as_tensor(geo, dtype=torch.int16)
+            assert tuple(g.shape) == (self.X, self.Y, self.Z)
+            self.map = to_storage(g).reshape(-1).to(self.device)
+        if hasattr(self, "iterations"):
+            self._bind()
+
+    def set_inflow(self, vx=0.0, vy=0.0, vz=0.0, profile=None):
+        """NSE_Data_ConstInflow values, or NSE_Data_InflowProfile arrays of shape (Z, Y)."""
+        self.d.inflow_vx, self.d.inflow_vy, self.d.inflow_vz = vx, vy, vz
+        self._profiles = [None, None, None]
+        for k in range(3):
+            if profile is not None and profile[k] is not None:
+                p = torch.as_tensor(profile[k], dtype=torch.float64, device=self.device).contiguous()
+                assert p.numel() == self.Y * self.Z
+                self._profiles[k] = p
+                self.d.inflow_profile[k] = p.data_ptr()
+            else:
+                self.d.inflow_profile[k] = None
+
+    # ---- initialisation -----------------------------------------------------
+    def set_equilibrium(self, rho=1.0, vx=0.0, vy=0.0, vz=0.0):
+        self._check(self.lib.tnl_lbm_d3q27_set_equilibrium(C.byref(self.cfg), C.byref(self.d), rho, vx, vy, vz,
+                                                           _stream_ptr()))
+
+    def set_equilibrium_field(self, rho, vx, vy, vz):
+        """Fields are [x, y, z] float64 arrays (host or device)."""
+        fs = [to_storage(torch.as_tensor(a, dtype=torch.float64)).reshape(-1).to(self.device) for a in (rho, vx, vy, vz)]
+        if self.d.XYZ != self.nsites:
+            raise ValueError("field initialisation expects XYZ == X*Y*Z")
+        self._check(self.lib.tnl_lbm_d3q27_set_equilibrium_field(C.byref(self.cfg), C.byref(self.d),
+                                                                 *[_ptr(f) for f in fs], _stream_ptr()))
+        torch.cuda.current_stream().synchronize()
+
+    def compute_macro(self):
+        self._check(self.lib.tnl_lbm_d3q27_compute_macro(C.byref(self.cfg), C.byref(self.d), _stream_ptr()))
+
+    # ---- the step -------------------------------------------------------------
+    def update_kernel_data(self):
+        """LBM::updateKernelData (lbm.hpp:307-323): A-A parity and A-B rotation."""
+        self.d.even_iter = 1 if self.iterations % 2 == 0 else 0
+        if self.cfg.pattern == _abi.PATTERN_AB:
+            cur = self.iterations % 2
+            self.d.df[0] = self.df[cur].data_ptr()
+            self.d.df[1] = self.df[1 - cur].data_ptr()
+
+    def step(self, output_macro=False, stream=None, flags=0, offset=(0, 0, 0), end=None):
+        self.update_kernel_data()
+        end = end or (self.X, self.Y, self.Z)
+        fl = flags | (_abi.FLAG_OUTPUT_MACRO if output_macro else 0)
+        self._check(self.lib.tnl_lbm_d3q27_step(C.byref(self.cfg), C.byref(self.d), _abi.I64x3(*offset),
+                                                _abi.I64x3(*end), fl, _stream_ptr(stream)))
+        self.iterations += 1
+
+    def current_df(self):
+        return self.iterations % 2 if self.cfg.pattern == _abi.PATTERN_AB else 0
+
+    # ---- readback -------------------------------------------------------------
+    def export_f64(self, which=None) -> torch.Tensor:
+        """Absolute populations, slot by slot, as a (27, X, Y, Z) float64 view."""
+        which = self.current_df() if which is None else which
+        out = torch.empty(27 * self.d.XYZ, dtype=torch.float64, device=self.device)
+        saved = self.d.df[0], self.d.df[1]
+        self.d.df[0] = self.df[which].data_ptr()
+        try:
+            self._check(self.lib.tnl_lbm_d3q27_export_f64(C.byref(self.cfg), C.byref(self.d), 0, _ptr(out),
+                                                          _stream_ptr()))
+        finally:
+            self.d.df[0], self.d.df[1] = saved
+        return out.view(27, self.d.XYZ)[:, :self.nsites].reshape(27, self.X, self.Z, self.Y).permute(0, 1, 3, 2)
+
+    def import_f64(self, f, which=0):
+        f = torch.as_tensor(f, dtype=torch.float64, device=self.device).permute(0, 1, 3, 2).contiguous()
+        if self.d.XYZ != self.nsites:
+            full = torch.zeros(27, self.d.XYZ, dtype=torch.float64, device=self.device)
+            full[:, :self.nsites] = f.reshape(27, -1)
+            f = full
+        f = f.reshape(-1)
+        self._check(self.lib.tnl_lbm_d3q27_import_f64(C.byref(self.cfg), C.byref(self.d), which, _ptr(f),
+                                                      _stream_ptr()))
+        torch.cuda.current_stream().synchronize()
+
+    def macros(self):
+        """rho, vx, vy, vz as [x, y, z] views of the macro array."""
+        m = self.macro.view(4, self.d.XYZ)[:, :self.nsites].view(4, self.X, self.Z, self.Y).permute(0, 1, 3, 2)
+        return m[0], m[1], m[2], m[3]
+
+    def reduce_macro(self):
+        """FP64 sums: mass, sum rho|u|^2, NaN count, sum (rho-1)^2, sum |u|^2, n."""
+        out = (C.c_double * 6)()
+        self._check(self.lib.tnl_lbm_d3q27_reduce_macro(C.byref(self.cfg), C.byref(self.d), out, _stream_ptr()))
+        return list(out)
