@@ -40,6 +40,12 @@ def parse():
     p.add_argument("--no-precision", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--storage", default="f64", choices=["f64", "f32", "f32dev", "f16dev", "bf16dev", "i16dev"],
+                   help="storage form of the headline case (f64 is the headline workload; the others exist "
+                        "so that each kernel can be profiled through this script)")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a plain device copy of 2 GiB (the known-size kernel that scales the "
+                        "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
 
 
@@ -63,12 +69,12 @@ def timed_steps(lat, steps, warmup):
     return wall, kern
 
 
-def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015):
+def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015, store_range=1.0):
     import torch
     from pytnl_lbm import D3Q27Lattice, _abi
     from pytnl_lbm.fields import turbulence_fields_torch
 
-    lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0)
+    lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0, store_range=store_range)
     u = turbulence_fields_torch(N, seed, k0, u_rms, device="cuda")
     rho = torch.ones_like(u[0])
     lat.set_equilibrium_field(rho, *u)
@@ -108,7 +114,7 @@ def precision_errors(N, steps, storages):
     nu = (0.505 - 0.5) / 3.0
 
     def run(storage):
-        lat = D3Q27Lattice(N, N, N, storage=storage, nu=nu)
+        lat = D3Q27Lattice(N, N, N, storage=storage, nu=nu, store_range=0.5)
         lat.set_equilibrium_field(*r)
         for t in range(steps):
             lat.step(output_macro=(t == steps - 1))
@@ -131,7 +137,7 @@ def precision_errors(N, steps, storages):
         e["u_L2"] = float(torch.linalg.vector_norm(du) / torch.linalg.vector_norm(ub))
         e["u_max"] = float(du.abs().max() / ub.abs().max())
         res[st] = e
-    return res, {"tgv": f"{N}^3", "U0": 0.05, "tau": 0.505, "steps": steps,
+    return res, {"tgv": f"{N}^3", "U0": 0.05, "tau": 0.505, "steps": steps, "i16_store_range": 0.5,
                  "rho_norm": "relative to rho-1 of the FP64 run"}
 
 
@@ -163,6 +169,42 @@ def cpu_baseline(seconds):
                       f"{threads} OpenMP threads"}
 
 
+def calibrate_copy(nbytes=2 << 30, reps=5):
+    """A plain device-to-device copy of known size: the FETCH_SIZE and
+    WRITE_SIZE counters read for it give the scale from counter units to
+    bytes."""
+    import torch
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps + 1,
+            "GBs": 2 * nbytes / dt / 1e9}
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the dominant kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the plain
+    copy), or None when no summary covers the workload."""
+    path = ROOT / "profiles" / "hbm_traffic.json"
+    if not path.exists():
+        return None
+    try:
+        table = json.loads(path.read_text())
+    except ValueError:
+        return None
+    row = table.get("workloads", {}).get(workload)
+    if not row:
+        return None
+    return row.get("bytes_per_launch")
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -172,8 +214,15 @@ def main():
     import torch
 
     torch.cuda.init()
+    if args.calibrate_copy:
+        print(json.dumps(calibrate_copy()), flush=True)
+        return
     N = args.size
-    head = bench_case(N, "f64", args.steps, args.warmup)
+    st = args.storage
+    tau = 0.5015 if st == "f64" else 0.505
+    head = bench_case(N, st, args.steps, args.warmup, tau=tau, store_range=0.5 if st == "i16dev" else 1.0)
+    kernel = f"k_aa_periodic<{st.upper()},CUM> (even/odd)"
+    workload = f"D3Q27 cumulant A-A {st.upper()}, {N}^3 periodic box, tau={tau}"
     out = {
         "metric": METRIC,
         "value": round(head["mlups"], 1),
@@ -185,13 +234,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if st == "f64" else "f32",
         "data": "synthetic: seeded solenoidal turbulence, E(k)~k^4 exp(-2(k/8)^2), u_rms=0.04, rho=1",
-        "config": {"workload": f"D3Q27 cumulant A-A FP64, {N}^3 periodic box, tau=0.5015", "lattice": [N, N, N],
-                   "collision": "CUM", "pattern": "AA", "storage": "f64"},
+        "config": {"workload": workload, "lattice": [N, N, N], "collision": "CUM", "pattern": "AA", "storage": st},
         "roofline": {"bound": "hbm", "achieved": round(head["achieved_GBs"], 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(head["roofline_frac"], 4), "traffic": None,
-                     "kernel": "k_aa_periodic<F64,CUM> (even/odd)", "bytes_per_site": head["bytes_per_site"],
+                     "unit": "GB/s", "frac": round(head["roofline_frac"], 4), "traffic": measured_traffic(workload),
+                     "kernel": kernel, "bytes_per_site": head["bytes_per_site"],
+                     "algorithmic_bytes_per_launch": head["bytes_per_site"] * N ** 3,
                      "kernel_ms_avg": round(head["kernel_ms_avg"], 3),
                      "kernel_ms_even": round(head["kernel_ms_even"], 3),
                      "kernel_ms_odd": round(head["kernel_ms_odd"], 3)},
@@ -200,14 +249,16 @@ def main():
     if not args.no_precision:
         per = {}
         P = args.prec_size
-        for st in ("f32", "f32dev", "f16dev", "bf16dev"):
-            r = bench_case(P, st, args.prec_steps, args.warmup)
-            per[f"{st}_{P}"] = {"mlups": round(r["mlups"], 1), "achieved_GBs": round(r["achieved_GBs"], 1),
-                                "roofline_frac": round(r["roofline_frac"], 4), "bytes_per_site": r["bytes_per_site"],
-                                "kernel_ms_avg": round(r["kernel_ms_avg"], 3)}
-        errs, spec = precision_errors(args.err_size, args.err_steps, ("f32", "f32dev", "f16dev", "bf16dev"))
-        for st, e in errs.items():
-            per[f"{st}_{P}"]["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in e.items()}
+        for pst in ("f32", "f32dev", "f16dev", "bf16dev", "i16dev"):
+            r = bench_case(P, pst, args.prec_steps, args.warmup, tau=0.505, store_range=0.5 if pst == "i16dev" else 1.0)
+            per[f"{pst}_{P}"] = {"mlups": round(r["mlups"], 1), "achieved_GBs": round(r["achieved_GBs"], 1),
+                                 "roofline_frac": round(r["roofline_frac"], 4), "bytes_per_site": r["bytes_per_site"],
+                                 "kernel_ms_avg": round(r["kernel_ms_avg"], 3),
+                                 "kernel_ms_even": round(r["kernel_ms_even"], 3),
+                                 "kernel_ms_odd": round(r["kernel_ms_odd"], 3)}
+        errs, spec = precision_errors(args.err_size, args.err_steps, ("f32", "f32dev", "f16dev", "bf16dev", "i16dev"))
+        for pst, e in errs.items():
+            per[f"{pst}_{P}"]["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in e.items()}
         out["per_precision"] = per
         out["precision_error_case"] = spec
     if not args.no_cpu:

@@ -101,6 +101,10 @@ enum tnl_lbm_d3q7_geo {
  * F32_DEV  : deviation f - w_q stored in float, FP32 arithmetic
  * F16_DEV  : deviation f - w_q stored in IEEE binary16, FP32 arithmetic
  * BF16_DEV : deviation f - w_q stored in bfloat16, FP32 arithmetic
+ * I16_DEV  : deviation as 16-bit fixed point: the signed-normalised integer
+ *            round(32767 * clamp((f - w_q) / (w_q * R), -1, 1)) with R =
+ *            store_range of the data struct; FP32 arithmetic.  Same 2 bytes
+ *            per population as FP16 with a uniform resolution of w_q R/32767.
  * The deviation forms follow the "well-conditioned" family of the reference
  * (d3q27/common_well.h:37, d3q27/eq_well.h:21-134), which is not wired into
  * any reference configuration; here it is a first-class storage form.      */
@@ -109,7 +113,8 @@ enum tnl_lbm_storage {
 	TNL_LBM_STORE_F32 = 1,
 	TNL_LBM_STORE_F32_DEV = 2,
 	TNL_LBM_STORE_F16_DEV = 3,
-	TNL_LBM_STORE_BF16_DEV = 4
+	TNL_LBM_STORE_BF16_DEV = 4,
+	TNL_LBM_STORE_I16_DEV = 5
 };
 
 /* ---- macro / field element types -------------------------------------- */
@@ -182,6 +187,7 @@ typedef struct tnl_lbm_d3q27_data {
 	/* NSE_Data_InflowProfile (lbm_data.h:100-132): double arrays indexed
 	 * Y*z + y; a non-NULL pointer overrides the constant component.        */
 	const double* inflow_profile[3];
+	double store_range;   /* I16_DEV only: R, the bound of |f - w_q| / w_q (> 0) */
 } tnl_lbm_d3q27_data;
 
 /* D3Q7 ADE data: ADE_Data (lbm_data.h:212-246) plus a source term that the

@@ -157,6 +157,17 @@ uint16_t f32_to_bf16(float v)
 	x += 0x7fffu + lsb;
 	return (uint16_t) (x >> 16);
 }
+// signed-normalised 16-bit: round(32767 * clamp(x, -1, 1)), ties to even
+int16_t f32_to_snorm16(float x)
+{
+	if (x != x)
+		return 0;
+	const float c = x < -1.0f ? -1.0f : (x > 1.0f ? 1.0f : x);
+	return (int16_t) std::nearbyint(c * 32767.0f);
+}
+// bound R of |f - w_q| / w_q for the I16 deviation form (set per oracle call)
+double g_store_range = 1.0;
+
 float bf16_to_f32(uint16_t h)
 {
 	const uint32_t x = (uint32_t) h << 16;
@@ -757,6 +768,27 @@ struct Store<TNL_LBM_STORE_BF16_DEV>
 	static real from_abs(double v, int q) { return (float) (v - w27(q)); }
 };
 
+template <>
+struct Store<TNL_LBM_STORE_I16_DEV>
+{
+	using real = float;
+	using elem = int16_t;
+	static constexpr bool dev = true;
+	// decode: raw * (w_q * R/32767); encode: snorm16(v * (1/w_q) * (1/R))
+	static real load(const elem* p, int64_t i, int q)
+	{
+		const float Rs = (float) (g_store_range / 32767.0);
+		return (float) p[i] * ((float) w27(q) * Rs);
+	}
+	static void store(elem* p, int64_t i, int q, real v)
+	{
+		const float invR = (float) (1.0 / g_store_range);
+		p[i] = f32_to_snorm16(v * ((float) (1.0 / w27(q)) * invR));
+	}
+	static double to_abs(real v, int q) { return (double) v + w27(q); }
+	static real from_abs(double v, int q) { return (float) (v - w27(q)); }
+};
+
 // Physics selector: absolute restatement or deviation form
 template <typename S>
 using Phys = std::conditional_t<S::dev, Dev, Abs<typename S::real>>;
@@ -1223,12 +1255,17 @@ struct Store7
 	// deviation forms: stored = (f_q - w_q phi_ref) * scale
 	static real load(const elem* p, int64_t i, int q, real phi_ref, real scale)
 	{
+		if constexpr (STORE == TNL_LBM_STORE_I16_DEV)
+			return ((float) p[i] * (float) (w7(q) / 32767.0)) / scale + (real) w7(q) * phi_ref;
 		const real v = S::load(p, i, q);
 		return S::dev ? v / scale + (real) w7(q) * phi_ref : v;
 	}
 	static void store(elem* p, int64_t i, int q, real v, real phi_ref, real scale)
 	{
-		S::store(p, i, q, S::dev ? (v - (real) w7(q) * phi_ref) * scale : v);
+		if constexpr (STORE == TNL_LBM_STORE_I16_DEV)
+			p[i] = f32_to_snorm16((v - (real) w7(q) * phi_ref) * scale * (float) (1.0 / w7(q)));
+		else
+			S::store(p, i, q, S::dev ? (v - (real) w7(q) * phi_ref) * scale : v);
 	}
 };
 
@@ -1400,8 +1437,11 @@ int check(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 		return fail(TNL_LBM_ERR_DIMS, "XYZ smaller than the stored lattice");
 	if (! SD->df[0] || (cfg->pattern == TNL_LBM_PATTERN_AB && ! SD->df[1]))
 		return fail(TNL_LBM_ERR_NULL, "population array missing");
-	if (cfg->storage < 0 || cfg->storage > 4 || cfg->pattern < 0 || cfg->pattern > 1)
+	if (cfg->storage < 0 || cfg->storage > 5 || cfg->pattern < 0 || cfg->pattern > 1)
 		return fail(TNL_LBM_ERR_ARG, "bad storage or pattern");
+	if (cfg->storage == TNL_LBM_STORE_I16_DEV && ! (SD->store_range > 0))
+		return fail(TNL_LBM_ERR_ARG, "store_range must be positive");
+	g_store_range = SD->store_range;
 	if (cfg->collision < 0 || cfg->collision > 2)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "collision operator not in the oracle");
 	const bool dev = cfg->storage >= TNL_LBM_STORE_F32_DEV;
@@ -1507,7 +1547,8 @@ int import_all(const tnl_lbm_d3q27_data* SD, int which, const double* in)
 		case TNL_LBM_STORE_F32: { constexpr int ST = TNL_LBM_STORE_F32; return CALL; } \
 		case TNL_LBM_STORE_F32_DEV: { constexpr int ST = TNL_LBM_STORE_F32_DEV; return CALL; } \
 		case TNL_LBM_STORE_F16_DEV: { constexpr int ST = TNL_LBM_STORE_F16_DEV; return CALL; } \
-		default: { constexpr int ST = TNL_LBM_STORE_BF16_DEV; return CALL; } \
+		case TNL_LBM_STORE_BF16_DEV: { constexpr int ST = TNL_LBM_STORE_BF16_DEV; return CALL; } \
+		default: { constexpr int ST = TNL_LBM_STORE_I16_DEV; return CALL; } \
 	}
 
 }  // namespace
@@ -1546,7 +1587,8 @@ int oracle_nse_ade_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* nse
 		case TNL_LBM_STORE_F32: CPL(TNL_LBM_STORE_F32)
 		case TNL_LBM_STORE_F32_DEV: CPL(TNL_LBM_STORE_F32_DEV)
 		case TNL_LBM_STORE_F16_DEV: CPL(TNL_LBM_STORE_F16_DEV)
-		default: CPL(TNL_LBM_STORE_BF16_DEV)
+		case TNL_LBM_STORE_BF16_DEV: CPL(TNL_LBM_STORE_BF16_DEV)
+		default: CPL(TNL_LBM_STORE_I16_DEV)
 	}
 #undef CPL
 }
@@ -1612,7 +1654,8 @@ int oracle_d3q7_set_equilibrium(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_da
 					case TNL_LBM_STORE_F32: ((float*) ADE->df[k])[q * ADE->XYZ + s] = (float) Ade<float>::eq(q, (float) p, 0, 0, 0); break;
 					case TNL_LBM_STORE_F32_DEV: ((float*) ADE->df[k])[q * ADE->XYZ + s] = dv; break;
 					case TNL_LBM_STORE_F16_DEV: ((uint16_t*) ADE->df[k])[q * ADE->XYZ + s] = f32_to_f16(dv); break;
-					default: ((uint16_t*) ADE->df[k])[q * ADE->XYZ + s] = f32_to_bf16(dv); break;
+					case TNL_LBM_STORE_BF16_DEV: ((uint16_t*) ADE->df[k])[q * ADE->XYZ + s] = f32_to_bf16(dv); break;
+					default: ((int16_t*) ADE->df[k])[q * ADE->XYZ + s] = f32_to_snorm16(dv * (float) (1.0 / w7(q))); break;
 				}
 			}
 		}

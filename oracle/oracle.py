@@ -102,16 +102,18 @@ def density_velocity(f, F=(0.0, 0.0, 0.0), single=False):
 
 # ---- whole lattice -----------------------------------------------------------
 _NP_ELEM = {_abi.STORE_F64: np.float64, _abi.STORE_F32: np.float32, _abi.STORE_F32_DEV: np.float32,
-            _abi.STORE_F16_DEV: np.uint16, _abi.STORE_BF16_DEV: np.uint16}
+            _abi.STORE_F16_DEV: np.uint16, _abi.STORE_BF16_DEV: np.uint16, _abi.STORE_I16_DEV: np.int16}
 
 
 class HostLattice:
     """A D3Q27 lattice in host memory driven by the oracle."""
 
     def __init__(self, X, Y, Z, *, collision=_abi.COLL_CUM, storage=_abi.STORE_F64, pattern=_abi.PATTERN_AA,
-                 macro_real=_abi.REAL_F64, nu=0.0, force=(0.0, 0.0, 0.0), geo=None, threads=None):
+                 macro_real=_abi.REAL_F64, nu=0.0, force=(0.0, 0.0, 0.0), geo=None, threads=None,
+                 store_range=1.0):
         self.cfg = _abi.Config.make(collision, storage, pattern, macro_real)
         self.d = _abi.D3Q27Data.make(X, Y, Z)
+        self.d.store_range = store_range
         self.X, self.Y, self.Z = X, Y, Z
         n = self.d.XYZ
         el = _NP_ELEM[storage]

@@ -176,3 +176,21 @@ def test_bf16_conversion_matches_torch():
         assert L.oracle_f32_to_bf16(float(v)) == r
     for r in ref[:200]:
         assert np.float32(L.oracle_bf16_to_f32(int(r))) == np.array([int(r) << 16], dtype=np.uint32).view(np.float32)[0]
+
+
+def test_snorm16_conversion_is_round_to_nearest_even():
+    """The fixed-point deviation form stores round(32767 * clamp(x, -1, 1))."""
+    import ctypes as C
+    L = O.lib()
+    rng = np.random.default_rng(8)
+    x = np.concatenate([rng.uniform(-1.2, 1.2, 5000), (np.arange(-20, 20) + 0.5) / 32767.0]).astype(np.float32)
+    lat = O.HostLattice(1, 1, 1, storage=5, store_range=1.0)
+    # encode through an import of absolute values f = w (1 + x) for the rest site
+    from lbm_common import W
+    for chunk in np.array_split(x, 200):
+        for v in chunk[:3]:
+            f = (W * (1.0 + np.float64(v))).reshape(27, 1, 1, 1)
+            lat.import_f64(f)
+            got = lat.df[0].reshape(27)[0]
+            want = np.int16(np.rint(np.float32(np.clip(np.float32(np.float32(f[0, 0, 0, 0] - W[0]) * np.float32((np.float32(1.0 / W[0])) * np.float32(1.0))), -1, 1)) * np.float32(32767.0)))
+            assert got == want

@@ -42,9 +42,11 @@ struct Ade7
 	using real = typename S::real;
 	using elem = typename S::elem;
 
+	// deviation forms store (f_q - w_q phi_ref) * scale; the fixed-point form
+	// stores that quantity divided by w_q as a signed-normalised integer
 	static LBM_DEV real ld(const elem* p, int q, real pr, real sc)
 	{
-		const real v = S::ld(p);
+		const real v = S::ld(p, (float) (w7(q) / 32767.0));
 		if constexpr (S::dev)
 			return v / sc + (real) w7(q) * pr;
 		else
@@ -53,7 +55,7 @@ struct Ade7
 	static LBM_DEV void st(elem* p, int q, real v, real pr, real sc)
 	{
 		if constexpr (S::dev)
-			S::st(p, (v - (real) w7(q) * pr) * sc);
+			S::st(p, (v - (real) w7(q) * pr) * sc, (float) (1.0 / w7(q)));
 		else
 			S::st(p, v);
 	}

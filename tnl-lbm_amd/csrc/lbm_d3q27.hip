@@ -45,6 +45,8 @@ struct StepArgs
 	int64_t X, Y, Z;     // sizes without overlaps
 	int64_t ox, oy, oz;  // overlaps
 	int64_t Ys, Zs;      // stored extents Y+2oy, Z+2oz
+	uint64_t plane_bytes;  // XYZ * sizeof(element): byte stride between population planes
+	float rs, invr;        // fixed-point form: R / 32767 and 1 / R
 	int32_t offx, offy, offz;
 	int32_t nx, ny, nz;
 	uint32_t nyb;      // y segments per row
@@ -123,32 +125,121 @@ LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real r
 // Even step (streaming_AA.h:81-84, 16-45): read own slots, write opposite.
 // Odd step (streaming_AA.h:86-115, 46-75): pull from r - c_q (opposite
 // slot), push to r + c_q (same slot).
+// Addresses: base + plane[q] (kernel argument, scalar) + row (scalar) + y
+// (per lane), one vector add per access.  The even step streams with
+// nontemporal hints; the odd step's +-1 shifted rows share cache lines
+// between neighbouring waves and keep the default policy.
 // ===========================================================================
+template <int ST, int COLL>
+LBM_DEV void collide_site(typename Storage<ST>::real* f, const StepArgs& A, typename Storage<ST>::real& rho,
+						  typename Storage<ST>::real& vx, typename Storage<ST>::real& vy, typename Storage<ST>::real& vz)
+{
+	using real = typename Storage<ST>::real;
+	if constexpr (std::is_same<real, float>::value && COLL == TNL_LBM_COLL_CUM) {
+		FastCum<Storage<ST>::dev>::collide(f, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, true, rho, vx, vy, vz);
+	}
+	else {
+		Phys<ST>::density_velocity(f, (real) A.fx, (real) A.fy, (real) A.fz, rho, vx, vy, vz);
+		collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
+	}
+}
+
+// Keeps `p` in scalar registers and makes everything derived from it depend
+// on `v`: the store addresses are then formed after the collision instead of
+// being hoisted above it (which would keep 27 scalar pairs live and spill).
+template <bool PIN, typename T>
+LBM_DEV char* after(char* p, const T& v)
+{
+	if constexpr (PIN)
+		asm volatile("" : "+s"(p) : "v"(v));
+	return p;
+}
+
+// Build-time tuning knobs (defaults are the measured best on MI355X):
+//  LBM_NT_EVEN   bitmask over storage forms: nontemporal hints on the even step
+//  LBM_PIN_ADDR  bitmask over storage forms: pin scalar address parts
+//  LBM_WAVES_F64 minimum waves per SIMD requested for the FP64 kernels
+#ifndef LBM_NT_EVEN
+#define LBM_NT_EVEN 0x1f
+#endif
+#ifndef LBM_PIN_ADDR
+#define LBM_PIN_ADDR 0x17
+#endif
+#ifndef LBM_WAVES_F64
+#define LBM_WAVES_F64 1
+#endif
+#ifndef LBM_MAXWAVES_F64
+#define LBM_MAXWAVES_F64 3
+#endif
+
+// Pins a wave-uniform offset in scalar registers so that the compiler cannot
+// re-associate it with the per-lane part into 64-bit vector multiply-adds.
+template <bool PIN>
+LBM_DEV uint64_t scalar(uint64_t v)
+{
+	if constexpr (PIN)
+		asm volatile("" : "+s"(v));
+	return v;
+}
+
 template <int ST, int COLL, bool EVEN>
-__global__ void __launch_bounds__(256) k_aa_periodic(const StepArgs A)
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(ST == TNL_LBM_STORE_F64 ? LBM_WAVES_F64 : 1, ST == TNL_LBM_STORE_F64 ? LBM_MAXWAVES_F64 : 8)))
+k_aa_periodic(const StepArgs A)
 {
 	using S = Storage<ST>;
 	using real = typename S::real;
 	using elem = typename S::elem;
+	constexpr uint32_t E = sizeof(elem);
+	constexpr bool NT = (LBM_NT_EVEN >> ST) & 1;
+	constexpr bool PIN = (LBM_PIN_ADDR >> ST) & 1;
 	int64_t x, z, y;
 	if (! block_coords(A, x, z, y))
 		return;
-	elem* __restrict__ df = (elem*) A.df0;
+	char* __restrict__ df = (char*) A.df0;
+	const uint64_t pb = A.plane_bytes;
 	real f[27];
 	const uint32_t yl = (uint32_t) (y + A.oy);
 	if constexpr (EVEN) {
-		const int64_t row = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys;
+		const uint64_t row = (uint64_t) (((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys) * E;
+		const uint32_t yb = yl * E;
 #pragma unroll
-		for (int q = 0; q < 27; q++)
-			f[q] = S::ld(at(df + (q * A.XYZ + row), yl));
+		for (int q = 0; q < 27; q++) {
+			const elem* p = (const elem*) (df + scalar<PIN>((uint64_t) q * pb + row) + yb);
+			const elem v = NT ? ld_nt(p) : *p;
+			f[q] = S::ld(&v, fx_sc(q, A.rs));
+		}
 		real rho, vx, vy, vz;
-		Phys<ST>::density_velocity(f, (real) A.fx, (real) A.fy, (real) A.fz, rho, vx, vy, vz);
-		collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
+		collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
+		char* out = after<PIN>(df, f[26]);
+		if constexpr (S::fixed) {
+			// two populations per conversion
 #pragma unroll
-		for (int q = 0; q < 27; q++)
-			S::st(at(df + (opp(q) * A.XYZ + row), yl), f[q]);
+			for (int q = 0; q < 27; q += 2) {
+				const int q1 = q + 1 < 27 ? q + 1 : q;
+				const auto r = __builtin_amdgcn_cvt_pknorm_i16(f[q] * fx_inv(q, A.invr), f[q1] * fx_inv(q1, A.invr));
+				elem* p0 = (elem*) (out + scalar<PIN>((uint64_t) opp(q) * pb + row) + yb);
+				NT ? st_nt(p0, (elem) r[0]) : (void) (*p0 = r[0]);
+				if (q + 1 < 27) {
+					elem* p1 = (elem*) (out + scalar<PIN>((uint64_t) opp(q1) * pb + row) + yb);
+					NT ? st_nt(p1, (elem) r[1]) : (void) (*p1 = r[1]);
+				}
+			}
+		}
+		else {
+#pragma unroll
+			for (int q = 0; q < 27; q++) {
+				elem v;
+				S::st(&v, f[q]);
+				elem* p = (elem*) (out + scalar<PIN>((uint64_t) opp(q) * pb + row) + yb);
+				if (NT)
+					st_nt(p, v);
+				else
+					*p = v;
+			}
+		}
 		if (A.out_macro)
-			write_macro(A, row + yl, rho, vx, vy, vz);
+			write_macro(A, (int64_t) (row / E) + yl, rho, vx, vy, vz);
 	}
 	else {
 		// periodic neighbours (kernelInitIndices, kernels.h:21-28)
@@ -160,19 +251,32 @@ __global__ void __launch_bounds__(256) k_aa_periodic(const StepArgs A)
 		const uint32_t yml = (uint32_t) (((! A.dist_y && y == 0) ? A.Y - 1 : y - 1) + A.oy);
 		const int64_t xs[3] = {xm + A.ox, x + A.ox, xp + A.ox};
 		const int64_t zs[3] = {zm + A.oz, z + A.oz, zp + A.oz};
-		const uint32_t ys[3] = {yml, yl, ypl};
+		const uint32_t ys[3] = {yml * E, yl * E, ypl * E};
 #pragma unroll
 		for (int q = 0; q < 27; q++) {
-			const int64_t row = (xs[1 - CX[q]] * A.Zs + zs[1 - CZ[q]]) * A.Ys;
-			f[q] = S::ld(at(df + (opp(q) * A.XYZ + row), ys[1 - CY[q]]));
+			const uint64_t row = (uint64_t) ((xs[1 - CX[q]] * A.Zs + zs[1 - CZ[q]]) * A.Ys) * E;
+			f[q] = S::ld((const elem*) (df + ((uint64_t) opp(q) * pb + row) + ys[1 - CY[q]]), fx_sc(opp(q), A.rs));
 		}
 		real rho, vx, vy, vz;
-		Phys<ST>::density_velocity(f, (real) A.fx, (real) A.fy, (real) A.fz, rho, vx, vy, vz);
-		collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
+		collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
+		char* out = after<PIN>(df, f[26]);
+		auto dst = [&](int q) {
+			const uint64_t row = (uint64_t) ((xs[1 + CX[q]] * A.Zs + zs[1 + CZ[q]]) * A.Ys) * E;
+			return (elem*) (out + scalar<PIN>((uint64_t) q * pb + row) + ys[1 + CY[q]]);
+		};
+		if constexpr (S::fixed) {
 #pragma unroll
-		for (int q = 0; q < 27; q++) {
-			const int64_t row = (xs[1 + CX[q]] * A.Zs + zs[1 + CZ[q]]) * A.Ys;
-			S::st(at(df + (q * A.XYZ + row), ys[1 + CY[q]]), f[q]);
+			for (int q = 0; q < 27; q += 2) {
+				if (q + 1 < 27)
+					S::st2(dst(q), f[q], fx_inv(q, A.invr), dst(q + 1), f[q + 1], fx_inv(q + 1, A.invr));
+				else
+					S::st(dst(q), f[q], fx_inv(q, A.invr));
+			}
+		}
+		else {
+#pragma unroll
+			for (int q = 0; q < 27; q++)
+				S::st(dst(q), f[q]);
 		}
 		if (A.out_macro)
 			write_macro(A, (xs[1] * A.Zs + zs[1]) * A.Ys + yl, rho, vx, vy, vz);
@@ -294,25 +398,25 @@ struct General
 				for (int q = 0; q < 27; q++) {
 					const int64_t yy = NY[-CY[q]], zz = NZ[-CZ[q]];
 					if (CX[q] < 0)
-						f[q] = cs * S::ld(cur + q * A.XYZ + sidx(A, NX.m, yy, zz)) + ((real) 1 - cs) * S::ld(cur + q * A.XYZ + sidx(A, NX.o, yy, zz));
+						f[q] = cs * S::ld(cur + q * A.XYZ + sidx(A, NX.m, yy, zz), fx_sc(q, A.rs)) + ((real) 1 - cs) * S::ld(cur + q * A.XYZ + sidx(A, NX.o, yy, zz), fx_sc(q, A.rs));
 					else
-						f[q] = S::ld(cur + q * A.XYZ + sidx(A, CX[q] > 0 ? NX.m : NX.o, yy, zz));
+						f[q] = S::ld(cur + q * A.XYZ + sidx(A, CX[q] > 0 ? NX.m : NX.o, yy, zz), fx_sc(q, A.rs));
 				}
 			}
 			else if (AA && A.even) {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX.o, NY.o, NZ.o));
+					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX.o, NY.o, NZ.o), fx_sc(q, A.rs));
 			}
 			else if (AA) {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + opp(q) * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+					f[q] = S::ld(cur + opp(q) * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(opp(q), A.rs));
 			}
 			else {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs));
 			}
 
 			const real rest = S::dev ? (real) 1 : (real) 0;
@@ -357,15 +461,15 @@ struct General
 						inflow(A, y, z, vx, vy, vz);
 						const real k2 = (real) 2 * (real) 3;
 						const int64_t so = sidx(A, NX.o, NY.o, NZ.o);
-						f[pzz] = S::ld(cur + mzz * A.XYZ + so) - k2 * (real) (2.0 / 27.0) * rho * (-vx);
-						f[ppz] = S::ld(cur + mmz * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vy);
-						f[pmz] = S::ld(cur + mpz * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vy);
-						f[pzp] = S::ld(cur + mzm * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vz);
-						f[pzm] = S::ld(cur + mzp * A.XYZ + so) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vz);
-						f[ppp] = S::ld(cur + mmm * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy - vz);
-						f[ppm] = S::ld(cur + mmp * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy + vz);
-						f[pmp] = S::ld(cur + mpm * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy - vz);
-						f[pmm] = S::ld(cur + mpp * A.XYZ + so) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy + vz);
+						f[pzz] = S::ld(cur + mzz * A.XYZ + so, fx_sc(mzz, A.rs)) - k2 * (real) (2.0 / 27.0) * rho * (-vx);
+						f[ppz] = S::ld(cur + mmz * A.XYZ + so, fx_sc(mmz, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vy);
+						f[pmz] = S::ld(cur + mpz * A.XYZ + so, fx_sc(mpz, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vy);
+						f[pzp] = S::ld(cur + mzm * A.XYZ + so, fx_sc(mzm, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vz);
+						f[pzm] = S::ld(cur + mzp * A.XYZ + so, fx_sc(mzp, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vz);
+						f[ppp] = S::ld(cur + mmm * A.XYZ + so, fx_sc(mmm, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy - vz);
+						f[ppm] = S::ld(cur + mmp * A.XYZ + so, fx_sc(mmp, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy + vz);
+						f[pmp] = S::ld(cur + mpm * A.XYZ + so, fx_sc(mpm, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy - vz);
+						f[pmm] = S::ld(cur + mpp * A.XYZ + so, fx_sc(mpp, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy + vz);
 						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
 						break;
 					}
@@ -446,19 +550,19 @@ struct General
 				elem* __restrict__ out = (elem*) A.df0;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					S::st(out + opp(q) * A.XYZ + s0, f[q]);
+					S::st(out + opp(q) * A.XYZ + s0, f[q], fx_inv(opp(q), A.invr));
 			}
 			else if (AA) {
 				elem* __restrict__ out = (elem*) A.df0;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					S::st(out + q * A.XYZ + sidx(A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q]);
+					S::st(out + q * A.XYZ + sidx(A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q], fx_inv(q, A.invr));
 			}
 			else {
 				elem* __restrict__ out = (elem*) A.df1;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					S::st(out + q * A.XYZ + s0, f[q]);
+					S::st(out + q * A.XYZ + s0, f[q], fx_inv(q, A.invr));
 			}
 		}
 		if (A.out_macro)
@@ -621,6 +725,7 @@ struct FieldArgs
 	const double* vy;
 	const double* vz;
 	double c[4];
+	float rs, invr;
 };
 
 template <int ST, int EQK>
@@ -642,12 +747,13 @@ __global__ void k_set_equilibrium(const FieldArgs A)
 		auto* out = (typename S::elem*) A.df[k];
 #pragma unroll
 		for (int q = 0; q < 27; q++)
-			S::st(out + q * A.XYZ + s, f[q]);
+			S::st(out + q * A.XYZ + s, f[q], fx_inv(q, A.invr));
 	}
 }
 
 struct MacroArgs
 {
+	float rs, invr;
 	const void* df;
 	void* macro;
 	int macro_f64;
@@ -669,7 +775,7 @@ __global__ void k_compute_macro(const MacroArgs A)
 	real f[27];
 #pragma unroll
 	for (int q = 0; q < 27; q++)
-		f[q] = S::ld((const typename S::elem*) A.df + q * A.XYZ + s);
+		f[q] = S::ld((const typename S::elem*) A.df + q * A.XYZ + s, fx_sc(q, A.rs));
 	real rho, vx, vy, vz;
 	Phys<ST>::density_velocity(f, 0, 0, 0, rho, vx, vy, vz);
 	if (A.macro_f64) {
@@ -683,7 +789,7 @@ __global__ void k_compute_macro(const MacroArgs A)
 }
 
 template <int ST, int Q, bool EXPORT>
-__global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites)
+__global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites, float rs, float invr)
 {
 	using S = Storage<ST>;
 	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -694,9 +800,9 @@ __global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites)
 	for (int q = 0; q < Q; q++) {
 		const double w = S::dev ? (Q == 27 ? w27(q) : (q == 0 ? 0.25 : 0.125)) : 0.0;
 		if constexpr (EXPORT)
-			buf[q * XYZ + s] = (double) S::ld(p + q * XYZ + s) + w;
+			buf[q * XYZ + s] = (double) S::ld(p + q * XYZ + s, fx_sc(q, rs)) + w;
 		else
-			S::st(p + q * XYZ + s, (typename S::real) (buf[q * XYZ + s] - w));
+			S::st(p + q * XYZ + s, (typename S::real) (buf[q * XYZ + s] - w), fx_inv(q, invr));
 	}
 }
 
@@ -785,8 +891,10 @@ int validate(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 		return fail(TNL_LBM_ERR_DIMS, "XYZ is smaller than the stored lattice");
 	if (SD->Y + 2 * SD->oy > (int64_t) 1 << 30)
 		return fail(TNL_LBM_ERR_DIMS, "Y extent too large");
-	if (cfg->storage < TNL_LBM_STORE_F64 || cfg->storage > TNL_LBM_STORE_BF16_DEV)
+	if (cfg->storage < TNL_LBM_STORE_F64 || cfg->storage > TNL_LBM_STORE_I16_DEV)
 		return fail(TNL_LBM_ERR_ARG, "unknown storage form");
+	if (cfg->storage == TNL_LBM_STORE_I16_DEV && ! (SD->store_range > 0))
+		return fail(TNL_LBM_ERR_ARG, "store_range must be positive for the fixed-point form");
 	if (cfg->pattern != TNL_LBM_PATTERN_AA && cfg->pattern != TNL_LBM_PATTERN_AB)
 		return fail(TNL_LBM_ERR_ARG, "unknown streaming pattern");
 	if (cfg->macro_real != TNL_LBM_REAL_F64 && cfg->macro_real != TNL_LBM_REAL_F32)
@@ -811,7 +919,7 @@ int hip_check(hipError_t e, const char* what)
 double omega1_of(int storage, double nu)
 {
 	if (storage == TNL_LBM_STORE_F64)
-		return 1.0 / (3.0 * nu + 0.5);
+		return 1.0 / (3.0 * nu + 0.5);  // FP64 arithmetic
 	volatile float n = (float) nu;
 	volatile float t = 3.0f * n;
 	volatile float d = t + 0.5f;
@@ -848,14 +956,26 @@ int dispatch_coll(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads
 	}
 }
 
-#define STORAGE_SWITCH(storage, EXPR)                                                      \
-	switch (storage) {                                                                     \
-		case TNL_LBM_STORE_F64: { constexpr int ST = TNL_LBM_STORE_F64; return EXPR; }         \
-		case TNL_LBM_STORE_F32: { constexpr int ST = TNL_LBM_STORE_F32; return EXPR; }         \
-		case TNL_LBM_STORE_F32_DEV: { constexpr int ST = TNL_LBM_STORE_F32_DEV; return EXPR; } \
-		case TNL_LBM_STORE_F16_DEV: { constexpr int ST = TNL_LBM_STORE_F16_DEV; return EXPR; } \
-		default: { constexpr int ST = TNL_LBM_STORE_BF16_DEV; return EXPR; }                   \
+#define STORAGE_SWITCH(storage, EXPR)                                                        \
+	switch (storage) {                                                                       \
+		case TNL_LBM_STORE_F64: { constexpr int ST = TNL_LBM_STORE_F64; return EXPR; }           \
+		case TNL_LBM_STORE_F32: { constexpr int ST = TNL_LBM_STORE_F32; return EXPR; }           \
+		case TNL_LBM_STORE_F32_DEV: { constexpr int ST = TNL_LBM_STORE_F32_DEV; return EXPR; }   \
+		case TNL_LBM_STORE_F16_DEV: { constexpr int ST = TNL_LBM_STORE_F16_DEV; return EXPR; }   \
+		case TNL_LBM_STORE_BF16_DEV: { constexpr int ST = TNL_LBM_STORE_BF16_DEV; return EXPR; } \
+		default: { constexpr int ST = TNL_LBM_STORE_I16_DEV; return EXPR; }                     \
 	}
+
+// fixed-point scale factors (only the I16 form uses them)
+float rs_of(const tnl_lbm_d3q27_data* SD)
+{
+	return (float) (SD->store_range / 32767.0);
+}
+float invr_of(const tnl_lbm_d3q27_data* SD)
+{
+	return (float) (1.0 / SD->store_range);
+}
+const int ELEM_BYTES[6] = {8, 4, 4, 2, 2, 2};
 
 int64_t stored_sites(const tnl_lbm_d3q27_data* SD)
 {
@@ -873,6 +993,8 @@ MacroArgs macro_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, co
 	M.ox = SD->ox; M.oy = SD->oy; M.oz = SD->oz;
 	M.Ys = SD->Y + 2 * SD->oy;
 	M.Zs = SD->Z + 2 * SD->oz;
+	M.rs = rs_of(SD);
+	M.invr = invr_of(SD);
 	return M;
 }
 
@@ -931,6 +1053,9 @@ static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 	A.ox = SD->ox; A.oy = SD->oy; A.oz = SD->oz;
 	A.Ys = SD->Y + 2 * SD->oy;
 	A.Zs = SD->Z + 2 * SD->oz;
+	A.plane_bytes = (uint64_t) SD->XYZ * (uint64_t) ELEM_BYTES[cfg->storage];
+	A.rs = rs_of(SD);
+	A.invr = invr_of(SD);
 	A.offx = (int32_t) offset[0]; A.offy = (int32_t) offset[1]; A.offz = (int32_t) offset[2];
 	A.nx = (int32_t) (end[0] - offset[0]); A.ny = (int32_t) (end[1] - offset[1]); A.nz = (int32_t) (end[2] - offset[2]);
 	if (A.nx == 0 || A.ny == 0 || A.nz == 0)
@@ -1072,6 +1197,8 @@ static int set_eq_impl(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, 
 	F.rho = rho; F.vx = vx; F.vy = vy; F.vz = vz;
 	for (int i = 0; i < 4; i++)
 		F.c[i] = c[i];
+	F.rs = rs_of(SD);
+	F.invr = invr_of(SD);
 	const unsigned grid = (unsigned) ((F.nsites + 255) / 256);
 	hipStream_t st = (hipStream_t) stream;
 	const bool cum = cfg->collision <= TNL_LBM_COLL_CUM_2017_ANTIALIAS;
@@ -1131,9 +1258,9 @@ static int transfer(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int
 	auto go = [&](auto stc) {
 		constexpr int ST = decltype(stc)::value;
 		if (exp)
-			hipLaunchKernelGGL((k_transfer<ST, 27, true>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n);
+			hipLaunchKernelGGL((k_transfer<ST, 27, true>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n, rs_of(SD), invr_of(SD));
 		else
-			hipLaunchKernelGGL((k_transfer<ST, 27, false>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n);
+			hipLaunchKernelGGL((k_transfer<ST, 27, false>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n, rs_of(SD), invr_of(SD));
 		return hip_check(hipGetLastError(), "transfer launch");
 	};
 	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));

@@ -53,14 +53,39 @@ constexpr double w27(int q)
 // Storage forms.  `real` is the arithmetic type, `elem` the stored type.
 template <int ST>
 struct Storage;
+// nontemporal (streaming) accesses of a stored element through its bit type
+template <typename E>
+struct Bits;
+template <>
+struct Bits<double> { using type = uint64_t; };
+template <>
+struct Bits<float> { using type = uint32_t; };
+template <>
+struct Bits<_Float16> { using type = uint16_t; };
+template <>
+struct Bits<__bf16> { using type = uint16_t; };
+template <typename E>
+LBM_DEV E ld_nt(const E* p)
+{
+	using B = typename Bits<E>::type;
+	return __builtin_bit_cast(E, __builtin_nontemporal_load((const B*) p));
+}
+template <typename E>
+LBM_DEV void st_nt(E* p, E v)
+{
+	using B = typename Bits<E>::type;
+	__builtin_nontemporal_store(__builtin_bit_cast(B, v), (B*) p);
+}
+
 template <>
 struct Storage<TNL_LBM_STORE_F64>
 {
 	using elem = double;
 	using real = double;
 	static constexpr bool dev = false;
-	static LBM_DEV real ld(const elem* p) { return *p; }
-	static LBM_DEV void st(elem* p, real v) { *p = v; }
+	static constexpr bool fixed = false;
+	static LBM_DEV real ld(const elem* p, float = 1) { return *p; }
+	static LBM_DEV void st(elem* p, real v, float = 1) { *p = v; }
 };
 template <>
 struct Storage<TNL_LBM_STORE_F32>
@@ -68,8 +93,9 @@ struct Storage<TNL_LBM_STORE_F32>
 	using elem = float;
 	using real = float;
 	static constexpr bool dev = false;
-	static LBM_DEV real ld(const elem* p) { return *p; }
-	static LBM_DEV void st(elem* p, real v) { *p = v; }
+	static constexpr bool fixed = false;
+	static LBM_DEV real ld(const elem* p, float = 1) { return *p; }
+	static LBM_DEV void st(elem* p, real v, float = 1) { *p = v; }
 };
 template <>
 struct Storage<TNL_LBM_STORE_F32_DEV>
@@ -77,8 +103,9 @@ struct Storage<TNL_LBM_STORE_F32_DEV>
 	using elem = float;
 	using real = float;
 	static constexpr bool dev = true;
-	static LBM_DEV real ld(const elem* p) { return *p; }
-	static LBM_DEV void st(elem* p, real v) { *p = v; }
+	static constexpr bool fixed = false;
+	static LBM_DEV real ld(const elem* p, float = 1) { return *p; }
+	static LBM_DEV void st(elem* p, real v, float = 1) { *p = v; }
 };
 template <>
 struct Storage<TNL_LBM_STORE_F16_DEV>
@@ -86,8 +113,9 @@ struct Storage<TNL_LBM_STORE_F16_DEV>
 	using elem = _Float16;
 	using real = float;
 	static constexpr bool dev = true;
-	static LBM_DEV real ld(const elem* p) { return (float) *p; }
-	static LBM_DEV void st(elem* p, real v) { *p = (_Float16) v; }
+	static constexpr bool fixed = false;
+	static LBM_DEV real ld(const elem* p, float = 1) { return (float) *p; }
+	static LBM_DEV void st(elem* p, real v, float = 1) { *p = (_Float16) v; }
 };
 template <>
 struct Storage<TNL_LBM_STORE_BF16_DEV>
@@ -95,9 +123,40 @@ struct Storage<TNL_LBM_STORE_BF16_DEV>
 	using elem = __bf16;
 	using real = float;
 	static constexpr bool dev = true;
-	static LBM_DEV real ld(const elem* p) { return (float) *p; }
-	static LBM_DEV void st(elem* p, real v) { *p = (__bf16) v; }
+	static constexpr bool fixed = false;
+	static LBM_DEV real ld(const elem* p, float = 1) { return (float) *p; }
+	static LBM_DEV void st(elem* p, real v, float = 1) { *p = (__bf16) v; }
 };
+// 16-bit fixed point: raw = round(32767 clamp(g / (w_q R), -1, 1)); the
+// caller passes sc = w_q R / 32767 (decode) and inv = 1 / (w_q R) (encode)
+template <>
+struct Storage<TNL_LBM_STORE_I16_DEV>
+{
+	using elem = int16_t;
+	using real = float;
+	static constexpr bool dev = true;
+	static constexpr bool fixed = true;
+	static LBM_DEV real ld(const elem* p, float sc) { return (float) *p * sc; }
+	static LBM_DEV void st(elem* p, real v, float inv) { *p = __builtin_amdgcn_cvt_pknorm_i16(v * inv, 0.0f)[0]; }
+	// two values, one conversion
+	static LBM_DEV void st2(elem* p0, real v0, float inv0, elem* p1, real v1, float inv1)
+	{
+		const auto r = __builtin_amdgcn_cvt_pknorm_i16(v0 * inv0, v1 * inv1);
+		*p0 = r[0];
+		*p1 = r[1];
+	}
+};
+template <>
+struct Bits<int16_t> { using type = uint16_t; };
+// per-direction scale factors of the fixed-point form (w_q R / 32767, 1/(w_q R))
+LBM_DEV float fx_sc(int q, float rs)
+{
+	return (float) w27(q) * rs;
+}
+LBM_DEV float fx_inv(int q, float invr)
+{
+	return (float) (1.0 / w27(q)) * invr;
+}
 
 // ---------------------------------------------------------------------------
 // Absolute form (F64 and F32 storage): the reference plugins.
@@ -543,6 +602,192 @@ struct Dev
 #pragma unroll
 			for (int j = 0; j < 3; j++)
 				bwd(T[i][j][0], T[i][j][1], T[i][j][2], Wf(i) * Wf(j), vz, vz2, g[qidx(i, j, 0)], g[qidx(i, j, 1)], g[qidx(i, j, 2)]);
+	}
+};
+
+// ---------------------------------------------------------------------------
+// Lean default cumulant for the FP32-arithmetic forms.
+//
+// With the default relaxation (col_cum.h:330-366) the post-collision state
+// depends only on rho, u, the forcing and the six 2nd-order central moments,
+// so the forward transform reduces to the ten raw moments of order <= 2
+// (instead of the full 27-moment chain), and the backward x-pass has known
+// zeros.  Same mathematics as Abs::cumulant<false,false> / Dev::cumulant.
+// DEV = true works on deviations g = f - w (all chain values carried as
+// deviations from their rest values); DEV = false on absolute f.
+template <bool DEV>
+struct FastCum
+{
+	using real = float;
+
+	static LBM_DEV real rcp(real x)
+	{
+		return __builtin_amdgcn_rcpf(x);
+	}
+
+	// deviation backward pass with rest-sum R0 (R0 = 0 for DEV = false)
+	static LBM_DEV void bwd(real k0, real k1, real k2, real R0, real v, real v2, real& dm, real& dz, real& dp)
+	{
+		const real full = DEV ? R0 + k0 : k0;
+		dz = k0 * (1.0f - v2) - 2.0f * v * k1 - k2;
+		if (DEV)
+			dz -= R0 * v2;
+		const real hm = 0.5f * (v2 - v), hp = 0.5f * (v2 + v);
+		dm = full * hm + k1 * (v - 0.5f) + 0.5f * k2;
+		dp = full * hp + k1 * (v + 0.5f) + 0.5f * k2;
+	}
+	// same with k1 = 0
+	static LBM_DEV void bwd02(real k0, real k2, real R0, real v, real v2, real& dm, real& dz, real& dp)
+	{
+		const real full = DEV ? R0 + k0 : k0;
+		dz = k0 * (1.0f - v2) - k2;
+		if (DEV)
+			dz -= R0 * v2;
+		dm = full * (0.5f * (v2 - v)) + 0.5f * k2;
+		dp = full * (0.5f * (v2 + v)) + 0.5f * k2;
+	}
+	// k0 = k2 = 0 (and R0 = 0)
+	static LBM_DEV void bwd1(real k1, real v, real& dm, real& dz, real& dp)
+	{
+		dz = -2.0f * v * k1;
+		dm = k1 * (v - 0.5f);
+		dp = k1 * (v + 0.5f);
+	}
+	// k2 = 0 (R0 = 0)
+	static LBM_DEV void bwd01(real k0, real k1, real v, real v2, real& dm, real& dz, real& dp)
+	{
+		dz = k0 * (1.0f - v2) - 2.0f * v * k1;
+		dm = k0 * (0.5f * (v2 - v)) + k1 * (v - 0.5f);
+		dp = k0 * (0.5f * (v2 + v)) + k1 * (v + 0.5f);
+	}
+
+	static constexpr float Wf(int a)
+	{
+		return (float) W1[a];
+	}
+	static constexpr float M(int i)
+	{
+		return i == 0 ? 1.0f : i == 1 ? 0.0f : (float) (1.0 / 3.0);
+	}
+
+	// one full step of the default cumulant: f (or g) in, post-collision out;
+	// also returns rho, u (pre-collision, force-shifted, as the macros)
+	static LBM_DEV void collide(real* f, real Fx, real Fy, real Fz, real omega1, bool do_collide, real& rho, real& vx,
+								real& vy, real& vz)
+	{
+		// z pass sums per (x, y) column: t = f- + f0 + f+, d = f+ - f-, s = f+ + f-
+		real t[3][3], d[3][3], sm[3][3];
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int b = 0; b < 3; b++) {
+				const real fm = f[qidx(a, b, 0)], fz = f[qidx(a, b, 1)], fp = f[qidx(a, b, 2)];
+				sm[a][b] = fp + fm;
+				d[a][b] = fp - fm;
+				t[a][b] = sm[a][b] + fz;
+			}
+		// x-collapsed and y-collapsed partial sums
+		real tx[3], ty[3];  // tx[a] = sum_b t[a][b]; ty[b] = sum_a t[a][b]
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+			tx[a] = (t[a][0] + t[a][2]) + t[a][1];
+#pragma unroll
+		for (int b = 0; b < 3; b++)
+			ty[b] = (t[0][b] + t[2][b]) + t[1][b];
+		const real m0 = (tx[0] + tx[2]) + tx[1];  // sum f (or sum g)
+		const real jx = tx[2] - tx[0];
+		const real Pxx = tx[2] + tx[0];
+		const real jy = ty[2] - ty[0];
+		const real Pyy = ty[2] + ty[0];
+		real dzs = 0, szs = 0;
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int b = 0; b < 3; b++) {
+				dzs += d[a][b];
+				szs += sm[a][b];
+			}
+		const real jz = dzs, Pzz = szs;
+		const real Pxy = (t[2][2] + t[0][0]) - (t[2][0] + t[0][2]);
+		const real Pxz = ((d[2][0] + d[2][1]) + d[2][2]) - ((d[0][0] + d[0][1]) + d[0][2]);
+		const real Pyz = ((d[0][2] + d[1][2]) + d[2][2]) - ((d[0][0] + d[1][0]) + d[2][0]);
+
+		rho = DEV ? m0 + 1.0f : m0;
+		const real rinv = rcp(rho);
+		vx = (jx + 0.5f * Fx) * rinv;
+		vy = (jy + 0.5f * Fy) * rinv;
+		vz = (jz + 0.5f * Fz) * rinv;
+		if (! do_collide)
+			return;
+		// 2nd-order central moments: k = P - u j - j u + rho u u  (j = rho u - F/2)
+		// (deviation form: the rest value 1/3 of the diagonal is already removed)
+		const real k200 = Pxx - vx * (2.0f * jx - rho * vx);
+		const real k020 = Pyy - vy * (2.0f * jy - rho * vy);
+		const real k002 = Pzz - vz * (2.0f * jz - rho * vz);
+		const real k110 = Pxy - vx * jy - vy * (jx - rho * vx);
+		const real k101 = Pxz - vx * jz - vz * (jx - rho * vx);
+		const real k011 = Pyz - vy * jz - vz * (jy - rho * vy);
+		const real third = (float) (1.0 / 3.0), ninth = (float) (1.0 / 9.0);
+		const real om = 1.0f - omega1;
+		const real Eq33 = om * (k200 - k020);
+		const real Eq34 = om * (k200 - k002);
+		const real Eq35 = DEV ? m0 : rho;  // k_000 * omega2 (omega2 = 1); DEV: deviation of rho
+		const real a = third * (Eq33 + Eq34 + Eq35);
+		const real b = third * (-2.0f * Eq33 + Eq34 + Eq35);
+		const real c = third * (Eq33 - 2.0f * Eq34 + Eq35);
+		const real s = om * k110, p = om * k101, qq = om * k011;
+		real ks211, ks121, ks112, ks220, ks022, ks202, ks222;
+		if (DEV) {
+			const real drho = m0;
+			ks211 = ((third + a) * qq + 2.0f * p * s) * rinv;
+			ks121 = ((third + b) * p + 2.0f * s * qq) * rinv;
+			ks112 = ((third + c) * s + 2.0f * qq * p) * rinv;
+			ks220 = ((b * a + 2.0f * s * s) + (b + a) * third - drho * ninth) * rinv;
+			ks022 = ((c * b + 2.0f * qq * qq) + (c + b) * third - drho * ninth) * rinv;
+			ks202 = ((a * c + 2.0f * p * p) + (a + c) * third - drho * ninth) * rinv;
+			const real X1 = (a + b + c) * ninth + (ks022 + ks202 + ks220) * third + a * ks022 + b * ks202 + c * ks220
+						  + 4.0f * (qq * ks211 + p * ks121 + s * ks112);
+			const real X2 = (float) (2.0 / 3.0) * ((a + b + c) * third + (a * b + b * c + c * a) + 2.0f * (p * p + qq * qq + s * s))
+						  + 16.0f * s * p * qq + 4.0f * (p * p * b + qq * qq * a + s * s * c) + 2.0f * a * b * c;
+			ks222 = (X1 - (X2 + drho * (drho - 1.0f) * (float) (1.0 / 27.0)) * rinv) * rinv;
+		}
+		else {
+			ks211 = (a * qq + 2.0f * p * s) * rinv;
+			ks121 = (b * p + 2.0f * s * qq) * rinv;
+			ks112 = (c * s + 2.0f * qq * p) * rinv;
+			ks220 = (b * a + 2.0f * s * s) * rinv;
+			ks022 = (c * b + 2.0f * qq * qq) * rinv;
+			ks202 = (a * c + 2.0f * p * p) * rinv;
+			ks222 = (a * ks022 + b * ks202 + c * ks220 + 4.0f * (qq * ks211 + p * ks121 + s * ks112)) * rinv
+				  - (16.0f * s * p * qq + 4.0f * (p * p * b + qq * qq * a + s * s * c) + 2.0f * a * b * c) * rinv * rinv;
+		}
+		const real k000 = DEV ? m0 : rho;
+		const real hx = 0.5f * Fx, hy = 0.5f * Fy, hz = 0.5f * Fz;  // ks_100 = -k_100 = F/2
+		const real vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+		// x pass: (j, k) columns of Ks[i][j][k]
+		real U[3][3][3];
+		bwd(k000, hx, a, M(0) * M(0), vx, vx2, U[0][0][0], U[1][0][0], U[2][0][0]);
+		bwd01(hy, s, vx, vx2, U[0][1][0], U[1][1][0], U[2][1][0]);
+		bwd02(b, ks220, M(2) * M(0), vx, vx2, U[0][2][0], U[1][2][0], U[2][2][0]);
+		bwd01(hz, p, vx, vx2, U[0][0][1], U[1][0][1], U[2][0][1]);
+		bwd02(qq, ks211, 0.0f, vx, vx2, U[0][1][1], U[1][1][1], U[2][1][1]);
+		bwd1(ks121, vx, U[0][2][1], U[1][2][1], U[2][2][1]);
+		bwd02(c, ks202, M(0) * M(2), vx, vx2, U[0][0][2], U[1][0][2], U[2][0][2]);
+		bwd1(ks112, vx, U[0][1][2], U[1][1][2], U[2][1][2]);
+		bwd02(ks022, ks222, M(2) * M(2), vx, vx2, U[0][2][2], U[1][2][2], U[2][2][2]);
+		// y pass
+		real T[3][3][3];
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int l = 0; l < 3; l++)
+				bwd(U[i][0][l], U[i][1][l], U[i][2][l], Wf(i) * M(l), vy, vy2, T[i][0][l], T[i][1][l], T[i][2][l]);
+		// z pass
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				bwd(T[i][j][0], T[i][j][1], T[i][j][2], Wf(i) * Wf(j), vz, vz2, f[qidx(i, j, 0)], f[qidx(i, j, 1)], f[qidx(i, j, 2)]);
 	}
 };
 

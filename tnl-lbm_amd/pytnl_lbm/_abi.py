@@ -27,9 +27,11 @@ STORE_F32 = 1
 STORE_F32_DEV = 2
 STORE_F16_DEV = 3
 STORE_BF16_DEV = 4
+STORE_I16_DEV = 5
 STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f32dev": STORE_F32_DEV,
-                 "f16dev": STORE_F16_DEV, "bf16dev": STORE_BF16_DEV}
-STORAGE_BYTES = {STORE_F64: 8, STORE_F32: 4, STORE_F32_DEV: 4, STORE_F16_DEV: 2, STORE_BF16_DEV: 2}
+                 "f16dev": STORE_F16_DEV, "bf16dev": STORE_BF16_DEV, "i16dev": STORE_I16_DEV}
+STORAGE_BYTES = {STORE_F64: 8, STORE_F32: 4, STORE_F32_DEV: 4, STORE_F16_DEV: 2, STORE_BF16_DEV: 2,
+                 STORE_I16_DEV: 2}
 
 # enum tnl_lbm_real
 REAL_F64 = 0
@@ -119,7 +121,7 @@ class D3Q27Data(C.Structure):
                 ("lbm_viscosity", C.c_double), ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double),
                 ("df", C.c_void_p * 2), ("macro", C.c_void_p), ("map", C.c_void_p),
                 ("inflow_vx", C.c_double), ("inflow_vy", C.c_double), ("inflow_vz", C.c_double),
-                ("inflow_profile", C.c_void_p * 3)]
+                ("inflow_profile", C.c_void_p * 3), ("store_range", C.c_double)]
 
     @classmethod
     def make(cls, X, Y, Z, ox=0, oy=0, oz=0, XYZ=None):
@@ -131,6 +133,7 @@ class D3Q27Data(C.Structure):
         d.XYZ = XYZ if XYZ is not None else (X + 2 * ox) * (Y + 2 * oy) * (Z + 2 * oz)
         d.even_iter = 1
         d.lbm_viscosity = 0.0
+        d.store_range = 1.0
         return d
 
 

@@ -19,7 +19,7 @@ from . import _abi
 from ._abi import GEO
 
 _TORCH_ELEM = {_abi.STORE_F64: torch.float64, _abi.STORE_F32: torch.float32, _abi.STORE_F32_DEV: torch.float32,
-               _abi.STORE_F16_DEV: torch.float16, _abi.STORE_BF16_DEV: torch.bfloat16}
+               _abi.STORE_F16_DEV: torch.float16, _abi.STORE_BF16_DEV: torch.bfloat16, _abi.STORE_I16_DEV: torch.int16}
 
 
 def _stream_ptr(stream=None):
@@ -48,7 +48,7 @@ class D3Q27Lattice:
 
     def __init__(self, X: int, Y: int, Z: int, *, collision="CUM", storage="f64", pattern="AA",
                  macro_real=None, nu: float = 0.0, force=(0.0, 0.0, 0.0), geo=None, device="cuda:0",
-                 XYZ: int | None = None):
+                 XYZ: int | None = None, store_range: float = 1.0):
         if not torch.cuda.is_available():
             raise RuntimeError("D3Q27Lattice needs a GPU (libtnl_lbm_amd.so runs on gfx950 only)")
         self.lib = _abi.load_library()
@@ -59,6 +59,7 @@ class D3Q27Lattice:
             macro_real = _abi.REAL_F64 if st == _abi.STORE_F64 else _abi.REAL_F32
         self.cfg = _abi.Config.make(coll, st, pat, macro_real)
         self.d = _abi.D3Q27Data.make(X, Y, Z, XYZ=XYZ)
+        self.d.store_range = store_range
         self.X, self.Y, self.Z = X, Y, Z
         self.device = torch.device(device)
         n = self.d.XYZ
