@@ -47,6 +47,10 @@ struct StepArgs
 	int64_t Ys, Zs;      // stored extents Y+2oy, Z+2oz
 	uint64_t plane_bytes;  // XYZ * sizeof(element): byte stride between population planes
 	float rs, invr;        // fixed-point form: R / 32767 and 1 / R
+	// fixed-point form per weight class n = |c_q|^2 (w_n = (2/3)^(3-n) / 6^n):
+	// decode w_n R / 32767, encode 1 / (w_n R) times the fast collision's
+	// 2^-n output factor
+	float cls_sc[4], cls_inv[4];
 	int32_t offx, offy, offz;
 	int32_t nx, ny, nz;
 	uint32_t nyb;      // y segments per row
@@ -131,11 +135,28 @@ LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real r
 // between neighbouring waves and keep the default policy.
 // ===========================================================================
 template <int ST, int COLL>
+constexpr bool fast_collision = std::is_same<typename Storage<ST>::real, float>::value && COLL == TNL_LBM_COLL_CUM;
+
+// number of non-zero components of c_q (the weight class of q)
+constexpr int ncls(int q)
+{
+	return (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+}
+
+// Factor applied to a post-collision value before its (non-fixed-point)
+// store: the fast collision leaves population q scaled by 2^n(q).
+template <int ST, int COLL>
+constexpr float out_factor(int q)
+{
+	return fast_collision<ST, COLL> ? FastCum<Storage<ST>::dev>::unscale(q) : 1.0f;
+}
+
+template <int ST, int COLL>
 LBM_DEV void collide_site(typename Storage<ST>::real* f, const StepArgs& A, typename Storage<ST>::real& rho,
 						  typename Storage<ST>::real& vx, typename Storage<ST>::real& vy, typename Storage<ST>::real& vz)
 {
 	using real = typename Storage<ST>::real;
-	if constexpr (std::is_same<real, float>::value && COLL == TNL_LBM_COLL_CUM) {
+	if constexpr (fast_collision<ST, COLL>) {
 		FastCum<Storage<ST>::dev>::collide(f, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, true, rho, vx, vy, vz);
 	}
 	else {
@@ -150,20 +171,27 @@ LBM_DEV void collide_site(typename Storage<ST>::real* f, const StepArgs& A, type
 template <bool PIN, typename T>
 LBM_DEV char* after(char* p, const T& v)
 {
-	if constexpr (PIN)
-		asm volatile("" : "+s"(p) : "v"(v));
+	if constexpr (PIN) {
+		auto g = (__attribute__((address_space(1))) char*) p;
+		asm volatile("" : "+s"(g) : "v"(v));
+		return (char*) g;
+	}
 	return p;
 }
 
 // Build-time tuning knobs (defaults are the measured best on MI355X):
 //  LBM_NT_EVEN   bitmask over storage forms: nontemporal hints on the even step
-//  LBM_PIN_ADDR  bitmask over storage forms: pin scalar address parts
+//  LBM_PIN_ADDR  bitmask over storage forms: scalar base pointers, even step
+//  LBM_PIN_ODD   bitmask over storage forms: scalar base pointers, odd step
 //  LBM_WAVES_F64 minimum waves per SIMD requested for the FP64 kernels
 #ifndef LBM_NT_EVEN
-#define LBM_NT_EVEN 0x1f
+#define LBM_NT_EVEN 0x3f
 #endif
 #ifndef LBM_PIN_ADDR
-#define LBM_PIN_ADDR 0x17
+#define LBM_PIN_ADDR 0x3f
+#endif
+#ifndef LBM_PIN_ODD
+#define LBM_PIN_ODD 0x00
 #endif
 #ifndef LBM_WAVES_F64
 #define LBM_WAVES_F64 1
@@ -172,14 +200,21 @@ LBM_DEV char* after(char* p, const T& v)
 #define LBM_MAXWAVES_F64 3
 #endif
 
-// Pins a wave-uniform offset in scalar registers so that the compiler cannot
-// re-associate it with the per-lane part into 64-bit vector multiply-adds.
+// Pins a wave-uniform pointer (base + plane + row) in scalar registers, so
+// that each access is one global load/store with a scalar base and a 32-bit
+// per-lane offset; left to itself the compiler adds the shared per-lane part
+// first and then pays a 64-bit vector add per access.
+// (The pinned copy is a global-address-space pointer: a plain pointer that
+// has passed through the asm would make every access a flat one.)
 template <bool PIN>
-LBM_DEV uint64_t scalar(uint64_t v)
+LBM_DEV char* sptr(char* p)
 {
-	if constexpr (PIN)
-		asm volatile("" : "+s"(v));
-	return v;
+	if constexpr (PIN) {
+		auto g = (__attribute__((address_space(1))) char*) p;
+		asm volatile("" : "+s"(g));
+		return (char*) g;
+	}
+	return p;
 }
 
 template <int ST, int COLL, bool EVEN>
@@ -193,6 +228,7 @@ k_aa_periodic(const StepArgs A)
 	constexpr uint32_t E = sizeof(elem);
 	constexpr bool NT = (LBM_NT_EVEN >> ST) & 1;
 	constexpr bool PIN = (LBM_PIN_ADDR >> ST) & 1;
+	constexpr bool PIN_ODD = (LBM_PIN_ODD >> ST) & 1;
 	int64_t x, z, y;
 	if (! block_coords(A, x, z, y))
 		return;
@@ -205,9 +241,9 @@ k_aa_periodic(const StepArgs A)
 		const uint32_t yb = yl * E;
 #pragma unroll
 		for (int q = 0; q < 27; q++) {
-			const elem* p = (const elem*) (df + scalar<PIN>((uint64_t) q * pb + row) + yb);
+			const elem* p = (const elem*) (sptr<PIN>(df + ((uint64_t) q * pb + row)) + yb);
 			const elem v = NT ? ld_nt(p) : *p;
-			f[q] = S::ld(&v, fx_sc(q, A.rs));
+			f[q] = S::ld(&v, A.cls_sc[ncls(q)]);
 		}
 		real rho, vx, vy, vz;
 		collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
@@ -217,11 +253,11 @@ k_aa_periodic(const StepArgs A)
 #pragma unroll
 			for (int q = 0; q < 27; q += 2) {
 				const int q1 = q + 1 < 27 ? q + 1 : q;
-				const auto r = __builtin_amdgcn_cvt_pknorm_i16(f[q] * fx_inv(q, A.invr), f[q1] * fx_inv(q1, A.invr));
-				elem* p0 = (elem*) (out + scalar<PIN>((uint64_t) opp(q) * pb + row) + yb);
+				const auto r = __builtin_amdgcn_cvt_pknorm_i16(f[q] * A.cls_inv[ncls(q)], f[q1] * A.cls_inv[ncls(q1)]);
+				elem* p0 = (elem*) (sptr<PIN>(out + ((uint64_t) opp(q) * pb + row)) + yb);
 				NT ? st_nt(p0, (elem) r[0]) : (void) (*p0 = r[0]);
 				if (q + 1 < 27) {
-					elem* p1 = (elem*) (out + scalar<PIN>((uint64_t) opp(q1) * pb + row) + yb);
+					elem* p1 = (elem*) (sptr<PIN>(out + ((uint64_t) opp(q1) * pb + row)) + yb);
 					NT ? st_nt(p1, (elem) r[1]) : (void) (*p1 = r[1]);
 				}
 			}
@@ -230,8 +266,8 @@ k_aa_periodic(const StepArgs A)
 #pragma unroll
 			for (int q = 0; q < 27; q++) {
 				elem v;
-				S::st(&v, f[q]);
-				elem* p = (elem*) (out + scalar<PIN>((uint64_t) opp(q) * pb + row) + yb);
+				S::st(&v, f[q] * out_factor<ST, COLL>(q));
+				elem* p = (elem*) (sptr<PIN>(out + ((uint64_t) opp(q) * pb + row)) + yb);
 				if (NT)
 					st_nt(p, v);
 				else
@@ -255,28 +291,28 @@ k_aa_periodic(const StepArgs A)
 #pragma unroll
 		for (int q = 0; q < 27; q++) {
 			const uint64_t row = (uint64_t) ((xs[1 - CX[q]] * A.Zs + zs[1 - CZ[q]]) * A.Ys) * E;
-			f[q] = S::ld((const elem*) (df + ((uint64_t) opp(q) * pb + row) + ys[1 - CY[q]]), fx_sc(opp(q), A.rs));
+			f[q] = S::ld((const elem*) (sptr<PIN_ODD>(df + ((uint64_t) opp(q) * pb + row)) + ys[1 - CY[q]]), A.cls_sc[ncls(q)]);
 		}
 		real rho, vx, vy, vz;
 		collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
-		char* out = after<PIN>(df, f[26]);
+		char* out = after<PIN_ODD>(df, f[26]);
 		auto dst = [&](int q) {
 			const uint64_t row = (uint64_t) ((xs[1 + CX[q]] * A.Zs + zs[1 + CZ[q]]) * A.Ys) * E;
-			return (elem*) (out + scalar<PIN>((uint64_t) q * pb + row) + ys[1 + CY[q]]);
+			return (elem*) (sptr<PIN_ODD>(out + ((uint64_t) q * pb + row)) + ys[1 + CY[q]]);
 		};
 		if constexpr (S::fixed) {
 #pragma unroll
 			for (int q = 0; q < 27; q += 2) {
 				if (q + 1 < 27)
-					S::st2(dst(q), f[q], fx_inv(q, A.invr), dst(q + 1), f[q + 1], fx_inv(q + 1, A.invr));
+					S::st2(dst(q), f[q], A.cls_inv[ncls(q)], dst(q + 1), f[q + 1], A.cls_inv[ncls(q + 1)]);
 				else
-					S::st(dst(q), f[q], fx_inv(q, A.invr));
+					S::st(dst(q), f[q], A.cls_inv[ncls(q)]);
 			}
 		}
 		else {
 #pragma unroll
 			for (int q = 0; q < 27; q++)
-				S::st(dst(q), f[q]);
+				S::st(dst(q), f[q] * out_factor<ST, COLL>(q));
 		}
 		if (A.out_macro)
 			write_macro(A, (xs[1] * A.Zs + zs[1]) * A.Ys + yl, rho, vx, vy, vz);
@@ -1056,6 +1092,16 @@ static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 	A.plane_bytes = (uint64_t) SD->XYZ * (uint64_t) ELEM_BYTES[cfg->storage];
 	A.rs = rs_of(SD);
 	A.invr = invr_of(SD);
+	{
+		const bool fast = cfg->storage != TNL_LBM_STORE_F64 && cfg->collision == TNL_LBM_COLL_CUM;
+		for (int n = 0; n < 4; n++) {
+			double w = 1.0;  // same product order as w27(): one value per class
+			for (int d = 0; d < 3; d++)
+				w *= d < n ? W1[0] : W1[1];
+			A.cls_sc[n] = (float) w * A.rs;
+			A.cls_inv[n] = (float) (1.0 / w) * A.invr * (fast ? 1.0f / (float) (1 << n) : 1.0f);
+		}
+	}
 	A.offx = (int32_t) offset[0]; A.offy = (int32_t) offset[1]; A.offz = (int32_t) offset[2];
 	A.nx = (int32_t) (end[0] - offset[0]); A.ny = (int32_t) (end[1] - offset[1]); A.nz = (int32_t) (end[2] - offset[2]);
 	if (A.nx == 0 || A.ny == 0 || A.nz == 0)

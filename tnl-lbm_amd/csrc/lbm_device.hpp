@@ -625,40 +625,60 @@ struct FastCum
 		return __builtin_amdgcn_rcpf(x);
 	}
 
-	// deviation backward pass with rest-sum R0 (R0 = 0 for DEV = false)
-	static LBM_DEV void bwd(real k0, real k1, real k2, real R0, real v, real v2, real& dm, real& dz, real& dp)
+	// Backward pass along one axis for a column (k0, k1, k2) of central
+	// moments with velocity v; R0 is the column's rest moment in the
+	// deviation form (a compile-time constant, unused when DEV = false).
+	// With  B = v full + k1  and  A = v (B + k1) + k2  (full = R0 + k0):
+	//   f_0 = k0 - A,   2 f_- = A - B,   2 f_+ = A + B,
+	// six operations per column.  The outer populations are left doubled;
+	// after three passes population q carries the factor 2^n(q), n(q) the
+	// number of non-zero components of c_q, which unscale(q) removes (the
+	// stores fold it into their own conversion or scaling).
+	static LBM_DEV void bwd(real k0, real k1, real k2, real R0, real v, real& dm, real& dz, real& dp)
 	{
 		const real full = DEV ? R0 + k0 : k0;
-		dz = k0 * (1.0f - v2) - 2.0f * v * k1 - k2;
-		if (DEV)
-			dz -= R0 * v2;
-		const real hm = 0.5f * (v2 - v), hp = 0.5f * (v2 + v);
-		dm = full * hm + k1 * (v - 0.5f) + 0.5f * k2;
-		dp = full * hp + k1 * (v + 0.5f) + 0.5f * k2;
+		const real B = v * full + k1;
+		const real A = v * (B + k1) + k2;
+		dz = k0 - A;
+		dm = A - B;
+		dp = A + B;
 	}
 	// same with k1 = 0
-	static LBM_DEV void bwd02(real k0, real k2, real R0, real v, real v2, real& dm, real& dz, real& dp)
+	static LBM_DEV void bwd02(real k0, real k2, real R0, real v, real& dm, real& dz, real& dp)
 	{
 		const real full = DEV ? R0 + k0 : k0;
-		dz = k0 * (1.0f - v2) - k2;
-		if (DEV)
-			dz -= R0 * v2;
-		dm = full * (0.5f * (v2 - v)) + 0.5f * k2;
-		dp = full * (0.5f * (v2 + v)) + 0.5f * k2;
+		const real B = v * full;
+		const real A = v * B + k2;
+		dz = k0 - A;
+		dm = A - B;
+		dp = A + B;
 	}
 	// k0 = k2 = 0 (and R0 = 0)
 	static LBM_DEV void bwd1(real k1, real v, real& dm, real& dz, real& dp)
 	{
-		dz = -2.0f * v * k1;
-		dm = k1 * (v - 0.5f);
-		dp = k1 * (v + 0.5f);
+		const real A = v * (k1 + k1);
+		dz = -A;
+		dm = A - k1;
+		dp = A + k1;
 	}
 	// k2 = 0 (R0 = 0)
-	static LBM_DEV void bwd01(real k0, real k1, real v, real v2, real& dm, real& dz, real& dp)
+	static LBM_DEV void bwd01(real k0, real k1, real v, real& dm, real& dz, real& dp)
 	{
-		dz = k0 * (1.0f - v2) - 2.0f * v * k1;
-		dm = k0 * (0.5f * (v2 - v)) + k1 * (v - 0.5f);
-		dp = k0 * (0.5f * (v2 + v)) + k1 * (v + 0.5f);
+		const real B = v * k0 + k1;
+		const real A = v * (B + k1);
+		dz = k0 - A;
+		dm = A - B;
+		dp = A + B;
+	}
+	// factor 2 carried by the outer positions of a pass, and its inverse
+	// over the three passes for population q
+	static constexpr float sig(int i)
+	{
+		return i == 1 ? 1.0f : 2.0f;
+	}
+	static constexpr float unscale(int q)
+	{
+		return 1.0f / (sig(CX[q] + 1) * sig(CY[q] + 1) * sig(CZ[q] + 1));
 	}
 
 	static constexpr float Wf(int a)
@@ -763,31 +783,31 @@ struct FastCum
 		}
 		const real k000 = DEV ? m0 : rho;
 		const real hx = 0.5f * Fx, hy = 0.5f * Fy, hz = 0.5f * Fz;  // ks_100 = -k_100 = F/2
-		const real vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
 		// x pass: (j, k) columns of Ks[i][j][k]
 		real U[3][3][3];
-		bwd(k000, hx, a, M(0) * M(0), vx, vx2, U[0][0][0], U[1][0][0], U[2][0][0]);
-		bwd01(hy, s, vx, vx2, U[0][1][0], U[1][1][0], U[2][1][0]);
-		bwd02(b, ks220, M(2) * M(0), vx, vx2, U[0][2][0], U[1][2][0], U[2][2][0]);
-		bwd01(hz, p, vx, vx2, U[0][0][1], U[1][0][1], U[2][0][1]);
-		bwd02(qq, ks211, 0.0f, vx, vx2, U[0][1][1], U[1][1][1], U[2][1][1]);
+		bwd(k000, hx, a, M(0) * M(0), vx, U[0][0][0], U[1][0][0], U[2][0][0]);
+		bwd01(hy, s, vx, U[0][1][0], U[1][1][0], U[2][1][0]);
+		bwd02(b, ks220, M(2) * M(0), vx, U[0][2][0], U[1][2][0], U[2][2][0]);
+		bwd01(hz, p, vx, U[0][0][1], U[1][0][1], U[2][0][1]);
+		bwd02(qq, ks211, 0.0f, vx, U[0][1][1], U[1][1][1], U[2][1][1]);
 		bwd1(ks121, vx, U[0][2][1], U[1][2][1], U[2][2][1]);
-		bwd02(c, ks202, M(0) * M(2), vx, vx2, U[0][0][2], U[1][0][2], U[2][0][2]);
+		bwd02(c, ks202, M(0) * M(2), vx, U[0][0][2], U[1][0][2], U[2][0][2]);
 		bwd1(ks112, vx, U[0][1][2], U[1][1][2], U[2][1][2]);
-		bwd02(ks022, ks222, M(2) * M(2), vx, vx2, U[0][2][2], U[1][2][2], U[2][2][2]);
-		// y pass
+		bwd02(ks022, ks222, M(2) * M(2), vx, U[0][2][2], U[1][2][2], U[2][2][2]);
+		// y pass (inputs at x position i carry sig(i))
 		real T[3][3][3];
 #pragma unroll
 		for (int i = 0; i < 3; i++)
 #pragma unroll
 			for (int l = 0; l < 3; l++)
-				bwd(U[i][0][l], U[i][1][l], U[i][2][l], Wf(i) * M(l), vy, vy2, T[i][0][l], T[i][1][l], T[i][2][l]);
-		// z pass
+				bwd(U[i][0][l], U[i][1][l], U[i][2][l], sig(i) * Wf(i) * M(l), vy, T[i][0][l], T[i][1][l], T[i][2][l]);
+		// z pass (inputs at (i, j) carry sig(i) sig(j))
 #pragma unroll
 		for (int i = 0; i < 3; i++)
 #pragma unroll
 			for (int j = 0; j < 3; j++)
-				bwd(T[i][j][0], T[i][j][1], T[i][j][2], Wf(i) * Wf(j), vz, vz2, f[qidx(i, j, 0)], f[qidx(i, j, 1)], f[qidx(i, j, 2)]);
+				bwd(T[i][j][0], T[i][j][1], T[i][j][2], sig(i) * sig(j) * Wf(i) * Wf(j), vz, f[qidx(i, j, 0)],
+					f[qidx(i, j, 1)], f[qidx(i, j, 2)]);
 	}
 };
 
