@@ -195,3 +195,81 @@ class HostLattice:
         """rho, vx, vy, vz as [x, y, z] arrays (float64)."""
         m = self.macro.astype(np.float64).reshape(4, self.X, self.Z, self.Y).transpose(0, 1, 3, 2)
         return m[0], m[1], m[2], m[3]
+
+    def step_coupled(self, ade, output_macro=False, flags=0):
+        """One coupled NSE + ADE step (kernels.h:114-196): parity and A-B rotation for both lattices."""
+        self.d.even_iter = 1 if self.iterations % 2 == 0 else 0
+        ade.d.even_iter = self.d.even_iter
+        if self.cfg.pattern == _abi.PATTERN_AB:
+            cur = self.iterations % 2
+            self.d.df[0] = self.df[cur].ctypes.data
+            self.d.df[1] = self.df[1 - cur].ctypes.data
+            ade.d.df[0] = ade.df[cur].ctypes.data
+            ade.d.df[1] = ade.df[1 - cur].ctypes.data
+        off = _abi.I64x3(0, 0, 0)
+        end = _abi.I64x3(self.X, self.Y, self.Z)
+        fl = flags | (_abi.FLAG_OUTPUT_MACRO if output_macro else 0)
+        _chk(lib().oracle_nse_ade_step(C.byref(self.cfg), C.byref(self.d), C.byref(ade.d), off, end, fl, self.threads))
+        self.iterations += 1
+
+
+class HostAde:
+    """D3Q7 ADE populations in host memory, stepped with their HostLattice."""
+
+    def __init__(self, nse: HostLattice, *, collision=_abi.ADE_COLL_SRT, diffusivity=0.0, geo=None, phi_ref=0.0,
+                 store_scale=1.0):
+        self.nse = nse
+        self.cfg = nse.cfg
+        self.X, self.Y, self.Z = nse.X, nse.Y, nse.Z
+        self.d = _abi.D3Q7Data.make(nse.X, nse.Y, nse.Z, XYZ=nse.d.XYZ)
+        self.d.collision = collision
+        self.d.lbm_diffusivity = diffusivity
+        self.d.phi_ref, self.d.store_scale = phi_ref, store_scale
+        n = self.d.XYZ
+        el = _NP_ELEM[self.cfg.storage]
+        self.df = [np.zeros(7 * n, dtype=el) for _ in range(2 if self.cfg.pattern == _abi.PATTERN_AB else 1)]
+        self._mdt = np.float64 if self.cfg.macro_real == _abi.REAL_F64 else np.float32
+        self.phi = np.zeros(n, dtype=self._mdt)
+        self.map = self.diffusion = self.source = None
+        self.d.df[0] = self.df[0].ctypes.data
+        self.d.df[1] = self.df[1].ctypes.data if len(self.df) > 1 else None
+        self.d.macro = self.phi.ctypes.data
+        if geo is not None:
+            self.set_geometry(geo)
+
+    def _field(self, a):
+        return np.ascontiguousarray(np.asarray(a, dtype=np.float64).transpose(0, 2, 1), dtype=self._mdt).reshape(-1)
+
+    def set_geometry(self, geo):
+        self.map = np.ascontiguousarray(np.asarray(geo).transpose(0, 2, 1).reshape(-1), dtype=np.int16)
+        self.d.map = self.map.ctypes.data
+
+    def set_diffusivity_field(self, D):
+        self.diffusion = self._field(D)
+        self.d.diffusion_coefficient = self.diffusion.ctypes.data
+
+    def set_source(self, S):
+        self.source = self._field(S)
+        self.d.source = self.source.ctypes.data
+
+    def set_inflow(self, phi):
+        self.d.inflow_phi = phi
+
+    def set_equilibrium(self, phi=0.0):
+        field = None if np.isscalar(phi) else self._field(phi)
+        _chk(lib().oracle_d3q7_set_equilibrium(C.byref(self.cfg), C.byref(self.d), _ptr(field),
+                                               float(phi) if field is None else 0.0))
+
+    def export_f64(self, which=None):
+        which = self.nse.current_df() if which is None else which
+        out = np.zeros(7 * self.d.XYZ)
+        saved = self.d.df[0], self.d.df[1]
+        self.d.df[0] = self.df[which].ctypes.data
+        try:
+            _chk(lib().oracle_d3q7_export_f64(C.byref(self.cfg), C.byref(self.d), 0, _ptr(out)))
+        finally:
+            self.d.df[0], self.d.df[1] = saved
+        return out.reshape(7, self.X, self.Z, self.Y).transpose(0, 1, 3, 2)
+
+    def phi_xyz(self):
+        return self.phi.astype(np.float64).reshape(self.X, self.Z, self.Y).transpose(0, 2, 1)

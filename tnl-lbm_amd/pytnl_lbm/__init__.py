@@ -8,8 +8,8 @@ nothing falls back to the CPU.
 """
 from . import _abi
 from ._abi import ADE_GEO, GEO, LBMError
-from .lattice import D3Q27Lattice, from_storage, to_storage
+from .lattice import D3Q7Lattice, D3Q27Lattice, from_storage, to_storage
 from .fields import tgv_fields, turbulence_fields
 
-__all__ = ["_abi", "GEO", "ADE_GEO", "LBMError", "D3Q27Lattice", "to_storage", "from_storage",
+__all__ = ["_abi", "GEO", "ADE_GEO", "LBMError", "D3Q27Lattice", "D3Q7Lattice", "to_storage", "from_storage",
            "tgv_fields", "turbulence_fields"]

@@ -58,6 +58,7 @@ COLLISION_NAMES = {"CUM": COLL_CUM, "CUM_2017": COLL_CUM_2017,
 ADE_COLL_SRT = 0
 ADE_COLL_MRT = 1
 ADE_COLL_CLBM = 2
+ADE_COLLISION_NAMES = {"SRT": ADE_COLL_SRT, "MRT": ADE_COLL_MRT, "CLBM": ADE_COLL_CLBM}
 
 FLAG_OUTPUT_MACRO = 0x1
 FLAG_DISTRIBUTED_X = 0x2

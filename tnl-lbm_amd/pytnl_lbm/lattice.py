@@ -153,6 +153,19 @@ class D3Q27Lattice:
                                                 _abi.I64x3(*end), fl, _stream_ptr(stream)))
         self.iterations += 1
 
+    def step_coupled(self, ade: "D3Q7Lattice", output_macro=False, stream=None, flags=0, offset=(0, 0, 0), end=None):
+        """One cudaLBMKernel<NSE,ADE> launch (kernels.h:114-196, state_NSE_ADE.h:201-210): the D3Q27
+        update and, at the same site, the D3Q7 update driven by the NSE velocity."""
+        if ade.nse is not self:
+            raise ValueError("the D3Q7 lattice belongs to another NSE lattice")
+        self.update_kernel_data()
+        ade.update_kernel_data(self.iterations)
+        end = end or (self.X, self.Y, self.Z)
+        fl = flags | (_abi.FLAG_OUTPUT_MACRO if output_macro else 0)
+        self._check(self.lib.tnl_lbm_nse_ade_step(C.byref(self.cfg), C.byref(self.d), C.byref(ade.d),
+                                                  _abi.I64x3(*offset), _abi.I64x3(*end), fl, _stream_ptr(stream)))
+        self.iterations += 1
+
     def current_df(self):
         return self.iterations % 2 if self.cfg.pattern == _abi.PATTERN_AB else 0
 
@@ -191,3 +204,113 @@ class D3Q27Lattice:
         out = (C.c_double * 6)()
         self._check(self.lib.tnl_lbm_d3q27_reduce_macro(C.byref(self.cfg), C.byref(self.d), out, _stream_ptr()))
         return list(out)
+
+
+class D3Q7Lattice:
+    """The advection-diffusion block of State_NSE_ADE (state_NSE_ADE.h:3-401):
+    D3Q7 populations in the storage form and streaming pattern of its NSE
+    lattice, stepped together with it by D3Q27Lattice.step_coupled.
+
+    ADE_Data (lbm_data.h:212-261): constant or per-site diffusivity D (the
+    relaxation time is 1/2 + 4D), D3Q7 geometry codes (ADE_GEO), an inflow
+    value, and a per-site source S added as w_q S after the collision.  The
+    deviation storage forms store (f_q - w_q phi_ref) * store_scale."""
+
+    def __init__(self, nse: D3Q27Lattice, *, collision="SRT", diffusivity: float = 0.0, geo=None,
+                 phi_ref: float = 0.0, store_scale: float = 1.0):
+        self.nse = nse
+        self.lib = nse.lib
+        self.cfg = nse.cfg
+        self.X, self.Y, self.Z = nse.X, nse.Y, nse.Z
+        self.device = nse.device
+        self.d = _abi.D3Q7Data.make(nse.X, nse.Y, nse.Z, XYZ=nse.d.XYZ)
+        self.d.collision = _abi.ADE_COLLISION_NAMES[collision] if isinstance(collision, str) else collision
+        self.d.lbm_diffusivity = diffusivity
+        self.d.phi_ref, self.d.store_scale = phi_ref, store_scale
+        n = self.d.XYZ
+        el = _TORCH_ELEM[self.cfg.storage]
+        ncop = 2 if self.cfg.pattern == _abi.PATTERN_AB else 1
+        self.df = [torch.empty(7 * n, dtype=el, device=self.device) for _ in range(ncop)]
+        self._mdt = torch.float64 if self.cfg.macro_real == _abi.REAL_F64 else torch.float32
+        self.phi = torch.zeros(n, dtype=self._mdt, device=self.device)
+        self.map = self.diffusion = self.source = None
+        if geo is not None:
+            self.set_geometry(geo)
+        self._bind()
+
+    @property
+    def nsites(self):
+        return self.X * self.Y * self.Z
+
+    def _bind(self):
+        self.d.df[0] = self.df[0].data_ptr()
+        self.d.df[1] = self.df[1].data_ptr() if len(self.df) > 1 else None
+        self.d.macro = self.phi.data_ptr()
+        self.d.map = self.map.data_ptr() if self.map is not None else None
+        self.d.diffusion_coefficient = self.diffusion.data_ptr() if self.diffusion is not None else None
+        self.d.source = self.source.data_ptr() if self.source is not None else None
+
+    def _field(self, a):
+        """[x, y, z] array -> flat device array in the lattice layout (macro_real)."""
+        t = torch.as_tensor(a, dtype=self._mdt)
+        assert tuple(t.shape) == (self.X, self.Y, self.Z)
+        flat = torch.zeros(self.d.XYZ, dtype=self._mdt, device=self.device)
+        flat[:self.nsites] = to_storage(t).reshape(-1).to(self.device)
+        return flat
+
+    def set_geometry(self, geo):
+        """geo: int16 [x, y, z] array of D3Q7 GEO codes (None = all periodic)."""
+        if geo is None:
+            self.map = None
+        else:
+            g = torch.as_tensor(geo, dtype=torch.int16)
+            assert tuple(g.shape) == (self.X, self.Y, self.Z)
+            self.map = to_storage(g).reshape(-1).to(self.device)
+        self._bind()
+
+    def set_diffusivity_field(self, D):
+        """Per-site diffusivity (ADE_Data::diffusionCoefficient, lbm_data.h:231-239); None = constant."""
+        self.diffusion = None if D is None else self._field(D)
+        self._bind()
+
+    def set_source(self, S):
+        """Per-site source S: every collided site gains w_q S (None = no source)."""
+        self.source = None if S is None else self._field(S)
+        self._bind()
+
+    def set_inflow(self, phi):
+        self.d.inflow_phi = phi
+
+    def set_equilibrium(self, phi=0.0):
+        """f_q = w_q phi (D3Q7 equilibrium at rest) at every stored site; phi is a scalar or an [x, y, z] field."""
+        field = None if isinstance(phi, (int, float)) else self._field(phi)
+        self._check(self.lib.tnl_lbm_d3q7_set_equilibrium(C.byref(self.cfg), C.byref(self.d), _ptr(field),
+                                                          float(phi) if field is None else 0.0, _stream_ptr()))
+        torch.cuda.current_stream().synchronize()
+
+    def update_kernel_data(self, iterations):
+        self.d.even_iter = 1 if iterations % 2 == 0 else 0
+        if self.cfg.pattern == _abi.PATTERN_AB:
+            cur = iterations % 2
+            self.d.df[0] = self.df[cur].data_ptr()
+            self.d.df[1] = self.df[1 - cur].data_ptr()
+
+    def _check(self, rc):
+        _abi.check(self.lib, rc)
+
+    def export_f64(self, which=None) -> torch.Tensor:
+        """Absolute populations, slot by slot, as a (7, X, Y, Z) float64 view."""
+        which = self.nse.current_df() if which is None else which
+        out = torch.empty(7 * self.d.XYZ, dtype=torch.float64, device=self.device)
+        saved = self.d.df[0], self.d.df[1]
+        self.d.df[0] = self.df[which].data_ptr()
+        try:
+            self._check(self.lib.tnl_lbm_d3q7_export_f64(C.byref(self.cfg), C.byref(self.d), 0, _ptr(out),
+                                                         _stream_ptr()))
+        finally:
+            self.d.df[0], self.d.df[1] = saved
+        return out.view(7, self.d.XYZ)[:, :self.nsites].reshape(7, self.X, self.Z, self.Y).permute(0, 1, 3, 2)
+
+    def phi_xyz(self):
+        """The scalar written on output steps, as an [x, y, z] view."""
+        return self.phi[:self.nsites].view(self.X, self.Z, self.Y).permute(0, 2, 1)
