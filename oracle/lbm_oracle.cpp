@@ -15,7 +15,11 @@
 // Parity pins (see tests/): closed-form equilibrium moments, collision
 // invariants and fixed points, 48-element cube-symmetry equivariance, shear
 // wave decay rate, A-A(S f0) == A-B(f0), deviation form == absolute form in
-// double, D3Q7 diffusion variance growth.  The reference has no golden
+// double, D3Q7 diffusion variance growth and advection, MRT_LES Smagorinsky
+// rate.  Parity unpinned beyond GPU == oracle: the BCs INFLOW_LEFT,
+// INFLOW_BB_LEFT, OUTFLOW_RIGHT_INTERP and SYM_*, the D3Q7 INFLOW /
+// OUTFLOW_RIGHT / SYM codes; partially pinned (through invariants only): the
+// 2017 cumulant limiters and the KBC stabiliser.  The reference has no golden
 // vectors for this path (its only test fixtures are unrelated IBM matrices
 // stored as Git-LFS pointers).
 
@@ -537,6 +541,328 @@ struct Abs
 		for (int q = 0; q < 27; q++)
 			f[q] = F[CX[q] + 1][CY[q] + 1][CZ[q] + 1];
 	}
+
+	// ---- D3Q27_EQ_ENTROPIC (eq_entropic.h:10-216) ------------------------
+	// rho * prod_d W(c_d) (2 - s_d) A_d^c_d with s_d = sqrt(1 + 3 u_d^2) and
+	// A_d = (2 u_d + s_d) / (1 - u_d); the reference multiplies the three
+	// weights, then the three (2 - s_d), then divides or multiplies by A_d
+	// axis by axis.
+	static dreal w1(int c)
+	{
+		return c == 0 ? n2o3 : (dreal) (1.0 / 6.0);
+	}
+	static dreal eq_entropic(int q, dreal rho, dreal vx, dreal vy, dreal vz)
+	{
+		const dreal v[3] = {vx, vy, vz};
+		const int c[3] = {CX[q], CY[q], CZ[q]};
+		dreal r = w1(c[0]) * w1(c[1]) * w1(c[2]);
+		for (int d = 0; d < 3; d++)
+			r = r * (no2 - std::sqrt(no1 + no3 * v[d] * v[d]));
+		for (int d = 0; d < 3; d++) {
+			const dreal A = (no2 * v[d] + std::sqrt(no1 + no3 * v[d] * v[d])) / (no1 - v[d]);
+			if (c[d] < 0)
+				r = r * no1 / A;
+			else if (c[d] > 0)
+				r = r * A;
+		}
+		return rho * r;
+	}
+	// equilibrium of a collision operator: 0 = EQ_INV_CUM, 1 = EQ, 2 = EQ_ENTROPIC
+	static dreal eq_any(int which, int q, dreal rho, dreal vx, dreal vy, dreal vz)
+	{
+		return which == 2 ? eq_entropic(q, rho, vx, vy, vz) : eq(which, q, rho, vx, vy, vz);
+	}
+	static void setEquilibriumAny(KS27<dreal>& KS, int which)
+	{
+		for (int q = 0; q < 27; q++)
+			KS.f[q] = eq_any(which, q, KS.rho, KS.vx, KS.vy, KS.vz);
+	}
+
+	// ---- moment sums in a fixed direction sequence -------------------------
+	struct Term
+	{
+		int8_t q, s;
+	};
+	template <size_t N>
+	static dreal msum(const dreal* f, const Term (&t)[N])
+	{
+		dreal r = t[0].s > 0 ? f[t[0].q] : -f[t[0].q];
+		for (size_t i = 1; i < N; i++)
+			r = t[i].s > 0 ? r + f[t[i].q] : r - f[t[i].q];
+		return r;
+	}
+	// directions in lexicographic order (x slowest; m < z < p on every axis)
+	static constexpr int LEX[27] = {mmm, mmz, mmp, mzm, mzz, mzp, mpm, mpz, mpp, zmm, zmz, zmp, zzm, zzz,
+									zzp, zpm, zpz, zpp, pmm, pmz, pmp, pzm, pzz, pzp, ppm, ppz, ppp};
+	// sum over LEX of c_a c_b f (zero terms skipped): the second-moment sums of col_mrt.h:16-29
+	static dreal lexsum2(const dreal* f, int a, int b)
+	{
+		bool first = true;
+		dreal r = 0;
+		for (int i = 0; i < 27; i++) {
+			const int q = LEX[i];
+			const int c[3] = {CX[q], CY[q], CZ[q]};
+			const int w = c[a] * c[b];
+			if (w == 0)
+				continue;
+			if (first)
+				r = w > 0 ? f[q] : -f[q];
+			else
+				r = w > 0 ? r + f[q] : r - f[q];
+			first = false;
+		}
+		return r;
+	}
+
+	// ---- D3Q27_MRT "MRT_LES" (col_mrt.h:7-142) ---------------------------
+	// Only the second moment Pi is relaxed, with a Smagorinsky rate
+	//   omega = 2 / (sqrt(tau^2 + 2 C 3 3 sqrt(Q) / rho) + tau),  C = 0.0342,
+	// Q = 2 |Pi_neq|^2; f is rebuilt from rho, u and Pi (2nd-order Hermite form).
+	static void mrt(KS27<dreal>& KS)
+	{
+		dreal P[6] = {lexsum2(KS.f, 0, 0), lexsum2(KS.f, 1, 1), lexsum2(KS.f, 2, 2),
+					  lexsum2(KS.f, 0, 1), lexsum2(KS.f, 0, 2), lexsum2(KS.f, 1, 2)};  // xx yy zz xy xz yz
+		const dreal rho = KS.rho, vx = KS.vx, vy = KS.vy, vz = KS.vz;
+		const dreal Pn[6] = {P[0] - rho * (n1o3 + vx * vx), P[1] - rho * (n1o3 + vy * vy), P[2] - rho * (n1o3 + vz * vz),
+							 P[3] - rho * vx * vy,          P[4] - rho * vx * vz,          P[5] - rho * vy * vz};
+		const dreal Q = no2 * (Pn[0] * Pn[0] + Pn[1] * Pn[1] + Pn[2] * Pn[2] + no2 * (Pn[3] * Pn[3] + Pn[4] * Pn[4] + Pn[5] * Pn[5]));
+		const dreal tau = no3 * KS.lbmViscosity + n1o2;
+		const dreal C = (dreal) 0.0342;
+		const dreal omega = no2 / (std::sqrt(tau * tau + no2 * C * no3 * no3 * std::sqrt(Q) / rho) + tau);
+		for (int k = 0; k < 6; k++)
+			P[k] -= omega * Pn[k];
+		for (int q = 0; q < 27; q++) {
+			const dreal cx = (dreal) CX[q], cy = (dreal) CY[q], cz = (dreal) CZ[q];
+			const int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+			const dreal w = nz == 0 ? n8o27 : nz == 1 ? n2o27 : nz == 2 ? n1o54 : n1o216;
+			KS.f[q] = w
+					* (rho * ((dreal) 2.5 - n3o2 * (dreal) nz + no3 * (vx * cx + vy * cy + vz * cz))
+					   + n9o2 * (P[0] * (cx * cx) + P[1] * (cy * cy) + P[2] * (cz * cz) + no2 * (P[3] * (cx * cy) + P[4] * (cx * cz) + P[5] * (cy * cz)))
+					   - n3o2 * (P[0] + P[1] + P[2]));
+		}
+	}
+
+	// ---- KBC N1-N4 / C1-C4 (col_kbc_n.h:254-1272, col_kbc_c.h:283-1301) ---
+	// f = k + s + h; the shear part s holds D (+T) (+Q) in raw (N) or central
+	// (C) moments.  beta = 1/(2 nu/cs^2 + 1); the stabiliser
+	//   gamma = 1/beta - (2 - 1/beta) <ds|dh> / <dh|dh>,  <x|y> = sum x y / feq;
+	// f -= beta (2 ds + gamma dh) - (1 - beta) S feq with the BGK-like force
+	// term S = 3 (c - u).F / rho.  feq is the product-form equilibrium.
+	// Moment sums follow the reference's direction sequences.
+	static constexpr Term M200_[18] = {{mmm, 1}, {mmp, 1}, {mmz, 1}, {mpm, 1}, {mpp, 1}, {mpz, 1}, {mzm, 1}, {mzp, 1}, {mzz, 1},
+									   {pmm, 1}, {pmp, 1}, {pmz, 1}, {ppm, 1}, {ppp, 1}, {ppz, 1}, {pzm, 1}, {pzp, 1}, {pzz, 1}};
+	static constexpr Term M020_[18] = {{mmm, 1}, {mmp, 1}, {mmz, 1}, {mpm, 1}, {mpp, 1}, {mpz, 1}, {zmm, 1}, {zmp, 1}, {zmz, 1},
+									   {zpm, 1}, {zpp, 1}, {zpz, 1}, {pmm, 1}, {pmp, 1}, {pmz, 1}, {ppm, 1}, {ppp, 1}, {ppz, 1}};
+	static constexpr Term M002_[18] = {{mmm, 1}, {mmp, 1}, {mzm, 1}, {mzp, 1}, {mpm, 1}, {mpp, 1}, {zmm, 1}, {zmp, 1}, {zzm, 1},
+									   {zzp, 1}, {zpm, 1}, {zpp, 1}, {pmm, 1}, {pmp, 1}, {pzm, 1}, {pzp, 1}, {ppm, 1}, {ppp, 1}};
+	static constexpr Term M110_[12] = {{mmm, 1},  {mmz, 1},  {mmp, 1},  {mpm, -1}, {mpz, -1}, {mpp, -1},
+									   {pmm, -1}, {pmz, -1}, {pmp, -1}, {ppm, 1},  {ppz, 1},  {ppp, 1}};
+	static constexpr Term M101_[12] = {{mmm, 1},  {mmp, -1}, {mzm, 1},  {mzp, -1}, {mpm, 1},  {mpp, -1},
+									   {pmm, -1}, {pmp, 1},  {pzm, -1}, {pzp, 1},  {ppm, -1}, {ppp, 1}};
+	static constexpr Term M011_[12] = {{mmm, 1}, {mmp, -1}, {mpm, -1}, {mpp, 1},  {zmm, 1},  {zmp, -1},
+									   {zpm, -1}, {zpp, 1}, {pmm, 1},  {pmp, -1}, {ppm, -1}, {ppp, 1}};
+	static constexpr Term M111_[8] = {{ppp, 1}, {mmp, 1}, {mpm, 1}, {mpp, -1}, {pmm, 1}, {pmp, -1}, {ppm, -1}, {mmm, -1}};
+	static constexpr Term M201_[12] = {{ppp, 1},  {mmp, 1}, {mzm, -1}, {mzp, 1}, {mpm, -1}, {mpp, 1},
+									   {pmm, -1}, {pmp, 1}, {pzm, -1}, {pzp, 1}, {ppm, -1}, {mmm, -1}};
+	static constexpr Term M102_[12] = {{ppp, 1}, {mmp, -1}, {mzm, -1}, {mzp, -1}, {mpm, -1}, {mpp, -1},
+									   {pmm, 1}, {pmp, 1},  {pzm, 1},  {pzp, 1},  {ppm, 1},  {mmm, -1}};
+	static constexpr Term M210_[12] = {{ppp, 1},  {mmz, -1}, {mmp, -1}, {mpm, 1}, {mpz, 1}, {mpp, 1},
+									   {pmm, -1}, {pmz, -1}, {pmp, -1}, {ppm, 1}, {ppz, 1}, {mmm, -1}};
+	static constexpr Term M120_[12] = {{ppp, 1}, {mmz, -1}, {mmp, -1}, {mpm, -1}, {mpz, -1}, {mpp, -1},
+									   {pmm, 1}, {pmz, 1},  {pmp, 1},  {ppm, 1},  {ppz, 1},  {mmm, -1}};
+	static constexpr Term M021_[12] = {{ppp, 1},  {mmp, 1},  {mpm, -1}, {mpp, 1}, {zmm, -1}, {zmp, 1},
+									   {zpm, -1}, {zpp, 1},  {pmm, -1}, {pmp, 1}, {ppm, -1}, {mmm, -1}};
+	static constexpr Term M012_[12] = {{ppp, 1},  {mmp, -1}, {mpm, 1},  {mpp, 1}, {zmm, -1}, {zmp, -1},
+									   {zpm, 1},  {zpp, 1},  {pmm, -1}, {pmp, -1}, {ppm, 1}, {mmm, -1}};
+
+	// value of one of the D, T, Q tensors at direction q; `div` selects the
+	// central-moment macros' divisions (x / 6) over the raw ones' products (x * 1/6)
+	static dreal scale(dreal x, dreal n, dreal inv, bool div)
+	{
+		return div ? x / n : x * inv;
+	}
+	static dreal tensD(int q, dreal Nxz, dreal Nyz, dreal Pxy, dreal Pxz, dreal Pyz, bool div)
+	{
+		const int cx = CX[q], cy = CY[q], cz = CZ[q];
+		const int nz = (cx != 0) + (cy != 0) + (cz != 0);
+		if (nz == 1) {
+			if (cx)
+				return scale(no2 * Nxz - Nyz, no6, (dreal) (1.0 / 6.0), div);
+			if (cy)
+				return scale(-Nxz + no2 * Nyz, no6, (dreal) (1.0 / 6.0), div);
+			return scale(-Nxz - Nyz, no6, (dreal) (1.0 / 6.0), div);
+		}
+		if (nz == 2) {
+			const dreal P = cz == 0 ? Pxy : cy == 0 ? Pxz : Pyz;
+			const int s = cz == 0 ? cx * cy : cy == 0 ? cx * cz : cy * cz;
+			return scale(s > 0 ? P : -P, no4, n1o4, div);
+		}
+		return 0;
+	}
+	static dreal tensT(int q, dreal T, bool div)
+	{
+		const int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+		if (nz == 0)
+			return -T;
+		return nz == 1 ? scale(T, no6, (dreal) (1.0 / 6.0), div) : (dreal) 0;
+	}
+	// Q components: xxy xxz xyy yyz xzz yzz xyz
+	static dreal tensQ(int q, const dreal* Qc, bool div)
+	{
+		const int cx = CX[q], cy = CY[q], cz = CZ[q];
+		const int nz = (cx != 0) + (cy != 0) + (cz != 0);
+		enum { xxy, xxz, xyy, yyz, xzz, yzz, xyz };
+		if (nz == 1) {
+			const dreal a = cx ? Qc[xyy] + Qc[xzz] : cy ? Qc[xxy] + Qc[yzz] : Qc[xxz] + Qc[yyz];
+			const int c = cx + cy + cz;
+			return scale(c > 0 ? -a : a, no2, n1o2, div);
+		}
+		if (nz == 2) {
+			dreal a, b;
+			int sa, sb;
+			if (cz == 0) {
+				a = Qc[xyy]; sa = cx; b = Qc[xxy]; sb = cy;
+			}
+			else if (cy == 0) {
+				a = Qc[xzz]; sa = cx; b = Qc[xxz]; sb = cz;
+			}
+			else {
+				a = Qc[yzz]; sa = cy; b = Qc[yyz]; sb = cz;
+			}
+			const dreal t = (sa > 0 ? a : -a);
+			return scale(sb > 0 ? t + b : t - b, no4, n1o4, div);
+		}
+		if (nz == 3)
+			return scale(cx * cy * cz > 0 ? Qc[xyz] : -Qc[xyz], no8, n1o8, div);
+		return 0;
+	}
+
+	template <bool CENTRAL, bool WT, bool WQ>
+	static void kbc(KS27<dreal>& KS)
+	{
+		const dreal rho = KS.rho, vx = KS.vx, vy = KS.vy, vz = KS.vz;
+		const dreal* f = KS.f;
+		// product-form equilibrium (col_kbc_n.h:289-330)
+		const dreal v[3] = {vx, vy, vz};
+		dreal Xs[3][3];  // [axis][c + 1]
+		for (int d = 0; d < 3; d++) {
+			const dreal z = n1o3 - no1 + v[d] * v[d];
+			const dreal p = -n1o2 * (z + no1 + v[d]);
+			Xs[d][0] = p + v[d];
+			Xs[d][1] = z;
+			Xs[d][2] = p;
+		}
+		dreal feq[27], ifeq[27];
+		for (int q = 0; q < 27; q++) {
+			feq[q] = -rho * Xs[0][CX[q] + 1] * Xs[1][CY[q] + 1] * Xs[2][CZ[q] + 1];
+			ifeq[q] = no1 / feq[q];
+		}
+		const dreal M200 = msum(f, M200_), M020 = msum(f, M020_), M002 = msum(f, M002_);
+		const dreal M110 = msum(f, M110_), M101 = msum(f, M101_), M011 = msum(f, M011_);
+		const dreal M111 = msum(f, M111_), M201 = msum(f, M201_), M102 = msum(f, M102_), M210 = msum(f, M210_);
+		const dreal M120 = msum(f, M120_), M021 = msum(f, M021_), M012 = msum(f, M012_);
+		const dreal rT = M200 + M020 + M002, rNxz = M200 - M002, rNyz = M020 - M002;
+		// tensor arguments of s and of its equilibrium
+		dreal T, Nxz, Nyz, Pxy, Pxz, Pyz, Qc[7];
+		dreal eT, eNxz, eNyz, ePxy, ePxz, ePyz, eQc[7];
+		if constexpr (! CENTRAL) {
+			T = rT; Nxz = rNxz; Nyz = rNyz; Pxy = M110; Pxz = M101; Pyz = M011;
+			const dreal q7[7] = {M210, M201, M120, M021, M102, M012, M111};
+			for (int k = 0; k < 7; k++)
+				Qc[k] = q7[k];
+			eT = rho * (no3 * n1o3 + vx * vx + vy * vy + vz * vz);
+			eNxz = rho * (vx * vx - vz * vz);
+			eNyz = rho * (vy * vy - vz * vz);
+			ePxy = rho * vx * vy;
+			ePxz = rho * vx * vz;
+			ePyz = rho * vy * vz;
+			eQc[0] = rho * vy * (n1o3 + vx * vx);
+			eQc[1] = rho * vz * (n1o3 + vx * vx);
+			eQc[2] = rho * vx * (n1o3 + vy * vy);
+			eQc[3] = rho * vz * (n1o3 + vy * vy);
+			eQc[4] = rho * vx * (n1o3 + vz * vz);
+			eQc[5] = rho * vy * (n1o3 + vz * vz);
+			eQc[6] = rho * vx * vy * vz;
+		}
+		else {
+			// central macros of col_kbc_c.h:57-69, kept as written (the cubic
+			// ones use 3 u^2 and u_x u_y u_z without the factor rho)
+			T = rT - rho * (vx * vx + vy * vy + vz * vz);
+			Nxz = rNxz + rho * (vz * vz - vx * vx);
+			Nyz = rNyz + rho * (vz * vz - vy * vy);
+			Pxy = M110 - rho * vx * vy;
+			Pxz = M101 - rho * vx * vz;
+			Pyz = M011 - rho * vy * vz;
+			Qc[0] = M210 - n1o3 * (no6 * vx * Pxy + vy * (no3 * vx * vx + no2 * Nxz - Nyz + T));
+			Qc[1] = M201 - n1o3 * (no6 * vx * Pxz + vz * (no3 * vx * vx + no2 * Nxz - Nyz + T));
+			Qc[2] = M120 - n1o3 * (no6 * vy * Pxy + vx * (no3 * vy * vy + no2 * Nyz - Nxz + T));
+			Qc[3] = M021 - n1o3 * (no6 * vy * Pyz + vz * (no3 * vy * vy + no2 * Nyz - Nxz + T));
+			Qc[4] = M102 - n1o3 * (no6 * vz * Pxz + vx * (no3 * vz * vz - Nyz - Nxz + T));
+			Qc[5] = M012 - n1o3 * (no6 * vz * Pyz + vy * (no3 * vz * vz - Nyz - Nxz + T));
+			Qc[6] = M111 - vx * Pyz - vy * Pxz - vz * Pxy - vx * vy * vz;
+			eT = rho * no3 * n1o3;
+			eNxz = eNyz = ePxy = ePxz = ePyz = 0;
+			for (int k = 0; k < 7; k++)
+				eQc[k] = 0;
+		}
+		dreal Ds[27], Dh[27];
+		for (int q = 0; q < 27; q++) {
+			dreal s = tensD(q, Nxz, Nyz, Pxy, Pxz, Pyz, CENTRAL) - tensD(q, eNxz, eNyz, ePxy, ePxz, ePyz, CENTRAL);
+			if constexpr (WT)
+				s = s + tensT(q, T, CENTRAL) - tensT(q, eT, CENTRAL);
+			if constexpr (WQ)
+				s = s + tensQ(q, Qc, CENTRAL) - tensQ(q, eQc, CENTRAL);
+			Ds[q] = s;
+			Dh[q] = f[q] - feq[q] - s;
+		}
+		const dreal beta = no1 / (no2 * KS.lbmViscosity / n1o3 + no1);
+		dreal num = 0, den = 0;
+		for (int i = 0; i < 27; i++) {
+			const int q = LEX[i];
+			num = i == 0 ? Ds[q] * Dh[q] * ifeq[q] : num + Ds[q] * Dh[q] * ifeq[q];
+			den = i == 0 ? Dh[q] * Dh[q] * ifeq[q] : den + Dh[q] * Dh[q] * ifeq[q];
+		}
+		const dreal gamma = no1 / beta - (no2 - no1 / beta) * num / den;
+		const dreal F[3] = {KS.fx, KS.fy, KS.fz};
+		for (int q = 0; q < 27; q++) {
+			const int c[3] = {CX[q], CY[q], CZ[q]};
+			dreal t[3];
+			for (int d = 0; d < 3; d++)
+				t[d] = c[d] == 0 ? -v[d] * F[d] : (c[d] > 0 ? (-v[d] + no1) * F[d] : (-v[d] - no1) * F[d]);
+			const dreal S = (no3 * (t[0] + t[1] + t[2])) / rho;
+			KS.f[q] -= beta * (no2 * Ds[q] + gamma * Dh[q]) - (no1 - beta) * S * feq[q];
+		}
+	}
+
+	// every absolute-form operator by id
+	static void collide_any(KS27<dreal>& KS, int coll)
+	{
+		switch (coll) {
+			case TNL_LBM_COLL_CUM: cumulant<false, false>(KS); break;
+			case TNL_LBM_COLL_CUM_2017: cumulant<true, false>(KS); break;
+			case TNL_LBM_COLL_CUM_2017_ANTIALIAS: cumulant<true, true>(KS); break;
+			case TNL_LBM_COLL_MRT_LES: mrt(KS); break;
+			case TNL_LBM_COLL_KBC_N1: kbc<false, false, false>(KS); break;
+			case TNL_LBM_COLL_KBC_N2: kbc<false, true, false>(KS); break;
+			case TNL_LBM_COLL_KBC_N3: kbc<false, false, true>(KS); break;
+			case TNL_LBM_COLL_KBC_N4: kbc<false, true, true>(KS); break;
+			case TNL_LBM_COLL_KBC_C1: kbc<true, false, false>(KS); break;
+			case TNL_LBM_COLL_KBC_C2: kbc<true, true, false>(KS); break;
+			case TNL_LBM_COLL_KBC_C3: kbc<true, false, true>(KS); break;
+			default: kbc<true, true, true>(KS); break;
+		}
+	}
+	// equilibrium each operator uses for initialisation and boundaries
+	// (the LBM_EQ template default of each plugin)
+	static int eq_of(int coll)
+	{
+		if (coll <= TNL_LBM_COLL_CUM_2017_ANTIALIAS)
+			return 0;
+		if (coll == TNL_LBM_COLL_MRT_LES || coll == TNL_LBM_COLL_KBC_N1 || coll == TNL_LBM_COLL_KBC_C1)
+			return 1;
+		return 2;
+	}
+
 };
 
 // ===========================================================================
@@ -829,18 +1155,29 @@ struct Site27
 
 	static int eq_kind()
 	{
-		return (COLL == TNL_LBM_COLL_CUM || COLL == TNL_LBM_COLL_CUM_2017 || COLL == TNL_LBM_COLL_CUM_2017_ANTIALIAS) ? 0 : 1;
+		return Abs<double>::eq_of(COLL);
+	}
+	// the operator's equilibrium (COLL::setEquilibrium, common.h:61-91)
+	static void set_eq(KS27<real>& KS)
+	{
+		if constexpr (S::dev)
+			P::setEquilibrium(KS, 0);
+		else
+			P::setEquilibriumAny(KS, eq_kind());
+	}
+	static real eqv(int q, real rho, real vx, real vy, real vz)
+	{
+		if constexpr (S::dev)
+			return P::eq(0, q, rho, vx, vy, vz);
+		else
+			return P::eq_any(eq_kind(), q, rho, vx, vy, vz);
 	}
 	static void collide(KS27<real>& KS)
 	{
 		if constexpr (S::dev)
 			P::cumulant(KS);
-		else if constexpr (COLL == TNL_LBM_COLL_CUM)
-			P::template cumulant<false, false>(KS);
-		else if constexpr (COLL == TNL_LBM_COLL_CUM_2017)
-			P::template cumulant<true, false>(KS);
 		else
-			P::template cumulant<true, true>(KS);
+			P::collide_any(KS, COLL);
 	}
 
 	// kernelInitIndices (kernels.h:5-57)
@@ -1008,7 +1345,7 @@ struct Site27
 			case GEO_INFLOW:
 				inflow(SD, KS, y, z);
 				KS.rho = 1;
-				P::setEquilibrium(KS, eq_kind());
+				set_eq(KS);
 				break;
 			case GEO_INFLOW_LEFT:
 				{
@@ -1061,12 +1398,12 @@ struct Site27
 				KS.rho = (real) 1.0 / (1 - KS.vx)
 					   * ((rest_sum + (f[zzz] + (+((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[zpz] + f[zmz]) + (f[zzp] + f[zzm])))))
 						  + 2 * (f[mzz] + (+((f[mpp] + f[mmm]) + (f[mpm] + f[mmp])) + ((f[mpz] + f[mmz]) + (f[mzp] + f[mzm])))));
-				P::setEquilibrium(KS, eq_kind());
+				set_eq(KS);
 				break;
 			case GEO_OUTFLOW_EQ:
 				P::densityVelocity(KS);
 				KS.rho = 1;
-				P::setEquilibrium(KS, eq_kind());
+				set_eq(KS);
 				break;
 			case GEO_OUTFLOW_RIGHT:
 				P::densityVelocity(KS);
@@ -1077,7 +1414,7 @@ struct Site27
 					// bc.h:195-200 with setEquilibriumDecomposition (common.h:94-124)
 					P::densityVelocity(KS);
 					for (int q = 0; q < 27; q++)
-						f[q] += P::eq(eq_kind(), q, (real) 1, KS.vx, KS.vy, KS.vz) - P::eq(eq_kind(), q, KS.rho, KS.vx, KS.vy, KS.vz);
+						f[q] += eqv(q, (real) 1, KS.vx, KS.vy, KS.vz) - eqv(q, KS.rho, KS.vx, KS.vy, KS.vz);
 					KS.rho = 1;
 					break;
 				}
@@ -1442,8 +1779,8 @@ int check(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 	if (cfg->storage == TNL_LBM_STORE_I16_DEV && ! (SD->store_range > 0))
 		return fail(TNL_LBM_ERR_ARG, "store_range must be positive");
 	g_store_range = SD->store_range;
-	if (cfg->collision < 0 || cfg->collision > 2)
-		return fail(TNL_LBM_ERR_UNSUPPORTED, "collision operator not in the oracle");
+	if (cfg->collision < TNL_LBM_COLL_CUM || cfg->collision > TNL_LBM_COLL_KBC_C4)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "unknown collision operator");
 	const bool dev = cfg->storage >= TNL_LBM_STORE_F32_DEV;
 	if (dev && cfg->collision != TNL_LBM_COLL_CUM)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "deviation storage supports the default cumulant only");
@@ -1454,11 +1791,27 @@ template <int STORE, int PATTERN>
 int dispatch_coll(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t* off, const int64_t* end,
 				  uint32_t flags, int nthreads)
 {
-	switch (cfg->collision) {
-		case TNL_LBM_COLL_CUM: return Site27<STORE, TNL_LBM_COLL_CUM, PATTERN>::run(SD, off, end, flags, cfg->macro_real, nthreads);
-		case TNL_LBM_COLL_CUM_2017: return Site27<STORE, TNL_LBM_COLL_CUM_2017, PATTERN>::run(SD, off, end, flags, cfg->macro_real, nthreads);
-		default: return Site27<STORE, TNL_LBM_COLL_CUM_2017_ANTIALIAS, PATTERN>::run(SD, off, end, flags, cfg->macro_real, nthreads);
+#define RUN(C) return Site27<STORE, C, PATTERN>::run(SD, off, end, flags, cfg->macro_real, nthreads)
+	if constexpr (Store<STORE>::dev) {
+		RUN(TNL_LBM_COLL_CUM);  // check() admits only the default cumulant here
 	}
+	else {
+		switch (cfg->collision) {
+			case TNL_LBM_COLL_CUM: RUN(TNL_LBM_COLL_CUM);
+			case TNL_LBM_COLL_CUM_2017: RUN(TNL_LBM_COLL_CUM_2017);
+			case TNL_LBM_COLL_CUM_2017_ANTIALIAS: RUN(TNL_LBM_COLL_CUM_2017_ANTIALIAS);
+			case TNL_LBM_COLL_MRT_LES: RUN(TNL_LBM_COLL_MRT_LES);
+			case TNL_LBM_COLL_KBC_N1: RUN(TNL_LBM_COLL_KBC_N1);
+			case TNL_LBM_COLL_KBC_N2: RUN(TNL_LBM_COLL_KBC_N2);
+			case TNL_LBM_COLL_KBC_N3: RUN(TNL_LBM_COLL_KBC_N3);
+			case TNL_LBM_COLL_KBC_N4: RUN(TNL_LBM_COLL_KBC_N4);
+			case TNL_LBM_COLL_KBC_C1: RUN(TNL_LBM_COLL_KBC_C1);
+			case TNL_LBM_COLL_KBC_C2: RUN(TNL_LBM_COLL_KBC_C2);
+			case TNL_LBM_COLL_KBC_C3: RUN(TNL_LBM_COLL_KBC_C3);
+			default: RUN(TNL_LBM_COLL_KBC_C4);
+		}
+	}
+#undef RUN
 }
 template <int STORE>
 int dispatch_pattern(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t* off, const int64_t* end,
@@ -1475,9 +1828,13 @@ void set_eq_site(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int64_
 	using S = Store<STORE>;
 	using P = Phys<S>;
 	using real = typename S::real;
-	const int which = cfg->collision <= 2 ? 0 : 1;
+	const int which = Abs<double>::eq_of(cfg->collision);
 	for (int q = 0; q < 27; q++) {
-		const real v = P::eq(which, q, (real) rho, (real) vx, (real) vy, (real) vz);
+		real v;
+		if constexpr (S::dev)
+			v = P::eq(0, q, (real) rho, (real) vx, (real) vy, (real) vz);
+		else
+			v = P::eq_any(which, q, (real) rho, (real) vx, (real) vy, (real) vz);
 		for (int k = 0; k < (cfg->pattern == TNL_LBM_PATTERN_AB ? 2 : 1); k++)
 			S::store((typename S::elem*) SD->df[k], q * SD->XYZ + s, q, v);
 	}
@@ -1696,16 +2053,11 @@ int oracle_d3q27_collide(int collision, int single, double* f, double nu, double
 		if (out) {
 			out[0] = KS.rho; out[1] = KS.vx; out[2] = KS.vy; out[3] = KS.vz;
 		}
-		if (collision == TNL_LBM_COLL_CUM)
-			Abs<real>::template cumulant<false, false>(KS);
-		else if (collision == TNL_LBM_COLL_CUM_2017)
-			Abs<real>::template cumulant<true, false>(KS);
-		else
-			Abs<real>::template cumulant<true, true>(KS);
+		Abs<real>::collide_any(KS, collision);
 		for (int q = 0; q < 27; q++)
 			f[q] = KS.f[q];
 	};
-	if (collision < 0 || collision > 2)
+	if (collision < TNL_LBM_COLL_CUM || collision > TNL_LBM_COLL_KBC_C4)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "collision");
 	if (single)
 		run(0.0f);
@@ -1743,8 +2095,8 @@ int oracle_d3q27_collide_dev(int collision, double* g, double nu, double fx, dou
 int oracle_d3q27_equilibrium(int which_eq, int single, double rho, double vx, double vy, double vz, double* f)
 {
 	for (int q = 0; q < 27; q++)
-		f[q] = single ? (double) Abs<float>::eq(which_eq, q, (float) rho, (float) vx, (float) vy, (float) vz)
-					  : Abs<double>::eq(which_eq, q, rho, vx, vy, vz);
+		f[q] = single ? (double) Abs<float>::eq_any(which_eq, q, (float) rho, (float) vx, (float) vy, (float) vz)
+					  : Abs<double>::eq_any(which_eq, q, rho, vx, vy, vz);
 	return TNL_LBM_OK;
 }
 

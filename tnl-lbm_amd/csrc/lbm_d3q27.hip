@@ -20,12 +20,11 @@
 #include <string>
 #include <type_traits>
 
-#include "lbm_ade.hpp"
-#include "lbm_device.hpp"
+#include "lbm_step.hpp"
 
 using namespace lbm;
 
-namespace {
+namespace lbm {
 
 thread_local std::string g_err;
 int fail(int code, const std::string& msg)
@@ -34,884 +33,17 @@ int fail(int code, const std::string& msg)
 	return code;
 }
 
-// ---------------------------------------------------------------------------
-struct StepArgs
+int hip_check(hipError_t e, const char* what)
 {
-	void* df0;
-	void* df1;
-	void* macro;
-	const int16_t* map;
-	int64_t XYZ;
-	int64_t X, Y, Z;     // sizes without overlaps
-	int64_t ox, oy, oz;  // overlaps
-	int64_t Ys, Zs;      // stored extents Y+2oy, Z+2oz
-	uint64_t plane_bytes;  // XYZ * sizeof(element): byte stride between population planes
-	float rs, invr;        // fixed-point form: R / 32767 and 1 / R
-	// fixed-point form per weight class n = |c_q|^2 (w_n = (2/3)^(3-n) / 6^n):
-	// decode w_n R / 32767, encode 1 / (w_n R) times the fast collision's
-	// 2^-n output factor
-	float cls_sc[4], cls_inv[4];
-	int32_t offx, offy, offz;
-	int32_t nx, ny, nz;
-	uint32_t nyb;      // y segments per row
-	uint32_t nblocks;  // logical blocks
-	uint32_t per_xcd;  // padded blocks / 8
-	int32_t even;
-	int32_t out_macro;
-	int32_t macro_f64;
-	int32_t dist_x, dist_y, dist_z;
-	double omega1;
-	double fx, fy, fz;
-	double inflow[3];
-	const double* prof[3];
-};
-
-constexpr int NXCD = 8;
-
-// logical block -> (x, z, y segment) with XCD-aware remapping
-LBM_DEV bool block_coords(const StepArgs& A, int64_t& x, int64_t& z, int64_t& y)
-{
-	const uint32_t b = blockIdx.x;
-	const uint32_t L = (b % NXCD) * A.per_xcd + b / NXCD;
-	if (L >= A.nblocks)
-		return false;
-	const uint32_t yb = L % A.nyb;
-	const uint32_t t = L / A.nyb;
-	z = A.offz + (int64_t) (t % (uint32_t) A.nz);
-	x = A.offx + (int64_t) (t / (uint32_t) A.nz);
-	y = A.offy + (int64_t) yb * blockDim.x + threadIdx.x;
-	return y < A.offy + A.ny;
+	if (e != hipSuccess)
+		return fail(TNL_LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+	return TNL_LBM_OK;
 }
 
-template <typename E>
-LBM_DEV E* at(E* base, uint32_t i)
-{
-	// uniform 64-bit base + 32-bit per-lane byte offset (global saddr form)
-	return (E*) ((char*) base + (uint32_t) (i * (uint32_t) sizeof(E)));
-}
+}  // namespace lbm
 
-template <typename real>
-LBM_DEV void write_macro(const StepArgs& A, int64_t s, real rho, real vx, real vy, real vz)
-{
-	if (A.macro_f64) {
-		double* m = (double*) A.macro;
-		m[s] = rho;
-		m[A.XYZ + s] = vx;
-		m[2 * A.XYZ + s] = vy;
-		m[3 * A.XYZ + s] = vz;
-	}
-	else {
-		float* m = (float*) A.macro;
-		m[s] = (float) rho;
-		m[A.XYZ + s] = (float) vx;
-		m[2 * A.XYZ + s] = (float) vy;
-		m[3 * A.XYZ + s] = (float) vz;
-	}
-}
+namespace {
 
-template <int ST, int COLL>
-LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real rho, typename Storage<ST>::real vx,
-					 typename Storage<ST>::real vy, typename Storage<ST>::real vz, typename Storage<ST>::real omega1)
-{
-	using P = Phys<ST>;
-	if constexpr (Storage<ST>::dev)
-		P::cumulant(f, rho, vx, vy, vz, omega1);
-	else if constexpr (COLL == TNL_LBM_COLL_CUM)
-		P::template cumulant<false, false>(f, rho, vx, vy, vz, omega1);
-	else if constexpr (COLL == TNL_LBM_COLL_CUM_2017)
-		P::template cumulant<true, false>(f, rho, vx, vy, vz, omega1);
-	else
-		P::template cumulant<true, true>(f, rho, vx, vy, vz, omega1);
-}
-
-// ===========================================================================
-// Fast path: A-A pattern, every site GEO_PERIODIC (no map).
-// Even step (streaming_AA.h:81-84, 16-45): read own slots, write opposite.
-// Odd step (streaming_AA.h:86-115, 46-75): pull from r - c_q (opposite
-// slot), push to r + c_q (same slot).
-// Addresses: base + plane[q] (kernel argument, scalar) + row (scalar) + y
-// (per lane), one vector add per access.  The even step streams with
-// nontemporal hints; the odd step's +-1 shifted rows share cache lines
-// between neighbouring waves and keep the default policy.
-// ===========================================================================
-template <int ST, int COLL>
-constexpr bool fast_collision = std::is_same<typename Storage<ST>::real, float>::value && COLL == TNL_LBM_COLL_CUM;
-
-// number of non-zero components of c_q (the weight class of q)
-constexpr int ncls(int q)
-{
-	return (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
-}
-
-// Factor applied to a post-collision value before its (non-fixed-point)
-// store: the fast collision leaves population q scaled by 2^n(q).
-template <int ST, int COLL>
-constexpr float out_factor(int q)
-{
-	return fast_collision<ST, COLL> ? FastCum<Storage<ST>::dev>::unscale(q) : 1.0f;
-}
-
-template <int ST, int COLL>
-LBM_DEV void collide_site(typename Storage<ST>::real* f, const StepArgs& A, typename Storage<ST>::real& rho,
-						  typename Storage<ST>::real& vx, typename Storage<ST>::real& vy, typename Storage<ST>::real& vz)
-{
-	using real = typename Storage<ST>::real;
-	if constexpr (fast_collision<ST, COLL>) {
-		FastCum<Storage<ST>::dev>::collide(f, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, true, rho, vx, vy, vz);
-	}
-	else {
-		Phys<ST>::density_velocity(f, (real) A.fx, (real) A.fy, (real) A.fz, rho, vx, vy, vz);
-		collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
-	}
-}
-
-// Keeps `p` in scalar registers and makes everything derived from it depend
-// on `v`: the store addresses are then formed after the collision instead of
-// being hoisted above it (which would keep 27 scalar pairs live and spill).
-template <bool PIN, typename T>
-LBM_DEV char* after(char* p, const T& v)
-{
-	if constexpr (PIN) {
-		auto g = (__attribute__((address_space(1))) char*) p;
-		asm volatile("" : "+s"(g) : "v"(v));
-		return (char*) g;
-	}
-	return p;
-}
-
-// Build-time tuning knobs (defaults are the measured best on MI355X):
-//  LBM_NT_EVEN   bitmask over storage forms: nontemporal hints on the even step
-//  LBM_PIN_ADDR  bitmask over storage forms: scalar base pointers, even step
-//  LBM_PIN_ODD   bitmask over storage forms: scalar base pointers, odd step
-//  LBM_WAVES_F64 minimum waves per SIMD requested for the FP64 kernels
-#ifndef LBM_NT_EVEN
-#define LBM_NT_EVEN 0x3f
-#endif
-#ifndef LBM_PIN_ADDR
-#define LBM_PIN_ADDR 0x3f
-#endif
-#ifndef LBM_PIN_ODD
-#define LBM_PIN_ODD 0x00
-#endif
-#ifndef LBM_WAVES_F64
-#define LBM_WAVES_F64 1
-#endif
-#ifndef LBM_MAXWAVES_F64
-#define LBM_MAXWAVES_F64 3
-#endif
-
-// Pins a wave-uniform pointer (base + plane + row) in scalar registers, so
-// that each access is one global load/store with a scalar base and a 32-bit
-// per-lane offset; left to itself the compiler adds the shared per-lane part
-// first and then pays a 64-bit vector add per access.
-// (The pinned copy is a global-address-space pointer: a plain pointer that
-// has passed through the asm would make every access a flat one.)
-template <bool PIN>
-LBM_DEV char* sptr(char* p)
-{
-	if constexpr (PIN) {
-		auto g = (__attribute__((address_space(1))) char*) p;
-		asm volatile("" : "+s"(g));
-		return (char*) g;
-	}
-	return p;
-}
-
-template <int ST, int COLL, bool EVEN>
-__global__ void __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(ST == TNL_LBM_STORE_F64 ? LBM_WAVES_F64 : 1, ST == TNL_LBM_STORE_F64 ? LBM_MAXWAVES_F64 : 8)))
-k_aa_periodic(const StepArgs A)
-{
-	using S = Storage<ST>;
-	using real = typename S::real;
-	using elem = typename S::elem;
-	constexpr uint32_t E = sizeof(elem);
-	constexpr bool NT = (LBM_NT_EVEN >> ST) & 1;
-	constexpr bool PIN = (LBM_PIN_ADDR >> ST) & 1;
-	constexpr bool PIN_ODD = (LBM_PIN_ODD >> ST) & 1;
-	int64_t x, z, y;
-	if (! block_coords(A, x, z, y))
-		return;
-	char* __restrict__ df = (char*) A.df0;
-	const uint64_t pb = A.plane_bytes;
-	real f[27];
-	const uint32_t yl = (uint32_t) (y + A.oy);
-	if constexpr (EVEN) {
-		const uint64_t row = (uint64_t) (((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys) * E;
-		const uint32_t yb = yl * E;
-#pragma unroll
-		for (int q = 0; q < 27; q++) {
-			const elem* p = (const elem*) (sptr<PIN>(df + ((uint64_t) q * pb + row)) + yb);
-			const elem v = NT ? ld_nt(p) : *p;
-			f[q] = S::ld(&v, A.cls_sc[ncls(q)]);
-		}
-		real rho, vx, vy, vz;
-		collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
-		char* out = after<PIN>(df, f[26]);
-		if constexpr (S::fixed) {
-			// two populations per conversion
-#pragma unroll
-			for (int q = 0; q < 27; q += 2) {
-				const int q1 = q + 1 < 27 ? q + 1 : q;
-				const auto r = __builtin_amdgcn_cvt_pknorm_i16(f[q] * A.cls_inv[ncls(q)], f[q1] * A.cls_inv[ncls(q1)]);
-				elem* p0 = (elem*) (sptr<PIN>(out + ((uint64_t) opp(q) * pb + row)) + yb);
-				NT ? st_nt(p0, (elem) r[0]) : (void) (*p0 = r[0]);
-				if (q + 1 < 27) {
-					elem* p1 = (elem*) (sptr<PIN>(out + ((uint64_t) opp(q1) * pb + row)) + yb);
-					NT ? st_nt(p1, (elem) r[1]) : (void) (*p1 = r[1]);
-				}
-			}
-		}
-		else {
-#pragma unroll
-			for (int q = 0; q < 27; q++) {
-				elem v;
-				S::st(&v, f[q] * out_factor<ST, COLL>(q));
-				elem* p = (elem*) (sptr<PIN>(out + ((uint64_t) opp(q) * pb + row)) + yb);
-				if (NT)
-					st_nt(p, v);
-				else
-					*p = v;
-			}
-		}
-		if (A.out_macro)
-			write_macro(A, (int64_t) (row / E) + yl, rho, vx, vy, vz);
-	}
-	else {
-		// periodic neighbours (kernelInitIndices, kernels.h:21-28)
-		const int64_t xp = (! A.dist_x && x == A.X - 1) ? 0 : x + 1;
-		const int64_t xm = (! A.dist_x && x == 0) ? A.X - 1 : x - 1;
-		const int64_t zp = (! A.dist_z && z == A.Z - 1) ? 0 : z + 1;
-		const int64_t zm = (! A.dist_z && z == 0) ? A.Z - 1 : z - 1;
-		const uint32_t ypl = (uint32_t) (((! A.dist_y && y == A.Y - 1) ? 0 : y + 1) + A.oy);
-		const uint32_t yml = (uint32_t) (((! A.dist_y && y == 0) ? A.Y - 1 : y - 1) + A.oy);
-		const int64_t xs[3] = {xm + A.ox, x + A.ox, xp + A.ox};
-		const int64_t zs[3] = {zm + A.oz, z + A.oz, zp + A.oz};
-		const uint32_t ys[3] = {yml * E, yl * E, ypl * E};
-#pragma unroll
-		for (int q = 0; q < 27; q++) {
-			const uint64_t row = (uint64_t) ((xs[1 - CX[q]] * A.Zs + zs[1 - CZ[q]]) * A.Ys) * E;
-			f[q] = S::ld((const elem*) (sptr<PIN_ODD>(df + ((uint64_t) opp(q) * pb + row)) + ys[1 - CY[q]]), A.cls_sc[ncls(q)]);
-		}
-		real rho, vx, vy, vz;
-		collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
-		char* out = after<PIN_ODD>(df, f[26]);
-		auto dst = [&](int q) {
-			const uint64_t row = (uint64_t) ((xs[1 + CX[q]] * A.Zs + zs[1 + CZ[q]]) * A.Ys) * E;
-			return (elem*) (sptr<PIN_ODD>(out + ((uint64_t) q * pb + row)) + ys[1 + CY[q]]);
-		};
-		if constexpr (S::fixed) {
-#pragma unroll
-			for (int q = 0; q < 27; q += 2) {
-				if (q + 1 < 27)
-					S::st2(dst(q), f[q], A.cls_inv[ncls(q)], dst(q + 1), f[q + 1], A.cls_inv[ncls(q + 1)]);
-				else
-					S::st(dst(q), f[q], A.cls_inv[ncls(q)]);
-			}
-		}
-		else {
-#pragma unroll
-			for (int q = 0; q < 27; q++)
-				S::st(dst(q), f[q] * out_factor<ST, COLL>(q));
-		}
-		if (A.out_macro)
-			write_macro(A, (xs[1] * A.Zs + zs[1]) * A.Ys + yl, rho, vx, vy, vz);
-	}
-}
-
-// ===========================================================================
-// General path: geometry map + D3Q27_BC_All, A-A or A-B.
-// ===========================================================================
-enum : int16_t
-{
-	GEO_FLUID = 0, GEO_WALL, GEO_INFLOW, GEO_INFLOW_LEFT, GEO_INFLOW_BB_LEFT, GEO_INFLOW_EQ_LEFT, GEO_OUTFLOW_EQ,
-	GEO_OUTFLOW_RIGHT, GEO_OUTFLOW_RIGHT_INTERP, GEO_PERIODIC, GEO_NOTHING, GEO_SYM_TOP, GEO_SYM_BOTTOM, GEO_SYM_LEFT,
-	GEO_SYM_RIGHT, GEO_SYM_BACK, GEO_SYM_FRONT
-};
-
-struct Nb
-{
-	int64_t m, o, p;  // storage coordinates (overlap-shifted) of c = -1, 0, +1
-	LBM_DEV int64_t operator[](int c) const { return c < 0 ? m : (c > 0 ? p : o); }
-};
-
-// kernelInitIndices (kernels.h:5-57).  A-A indices of non-periodic sites
-// that would leave the stored array wrap instead (the reference reads
-// outside its allocation there).
-template <bool AA>
-LBM_DEV Nb neighbours(int64_t c, int64_t n, int64_t o, bool dist, bool periodic)
-{
-	int64_t p, m;
-	if (periodic) {
-		p = (! dist && c == n - 1) ? 0 : c + 1;
-		m = (! dist && c == 0) ? n - 1 : c - 1;
-	}
-	else if (AA) {
-		p = c + 1 < n + o ? c + 1 : 0;
-		m = c - 1 >= -o ? c - 1 : n - 1;
-	}
-	else {
-		p = c + 1 < n - 1 + o ? c + 1 : n - 1 + o;
-		m = c - 1 > -o ? c - 1 : -o;
-	}
-	return Nb{m + o, c + o, p + o};
-}
-
-template <int ST, int COLL, int PATTERN>
-struct General
-{
-	using S = Storage<ST>;
-	using real = typename S::real;
-	using elem = typename S::elem;
-	using P = Phys<ST>;
-	static constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
-	static constexpr int EQK = (COLL <= TNL_LBM_COLL_CUM_2017_ANTIALIAS) ? 0 : 1;
-
-	static LBM_DEV int64_t sidx(const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
-	{
-		return (xs * A.Zs + zs) * A.Ys + ys;
-	}
-
-	static LBM_DEV void set_eq(real* f, real rho, real vx, real vy, real vz)
-	{
-#pragma unroll
-		for (int q = 0; q < 27; q++)
-			f[q] = eq_q(q, rho, vx, vy, vz);
-	}
-	static LBM_DEV real eq_q(int q, real rho, real vx, real vy, real vz)
-	{
-		// q is a compile-time constant after unrolling
-		switch (q) {
-#define EQC(Q) case Q: return P::template eq<EQK, Q>(rho, vx, vy, vz);
-			EQC(0) EQC(1) EQC(2) EQC(3) EQC(4) EQC(5) EQC(6) EQC(7) EQC(8) EQC(9) EQC(10) EQC(11) EQC(12) EQC(13)
-			EQC(14) EQC(15) EQC(16) EQC(17) EQC(18) EQC(19) EQC(20) EQC(21) EQC(22) EQC(23) EQC(24) EQC(25) EQC(26)
-#undef EQC
-		}
-		return 0;
-	}
-
-	static LBM_DEV void inflow(const StepArgs& A, int64_t y, int64_t z, real& vx, real& vy, real& vz)
-	{
-		// NSE_Data_ConstInflow / NSE_Data_InflowProfile::inflow (lbm_data.h:85-132)
-		const int64_t i = A.Y * z + y;
-		vx = (real) (A.prof[0] ? A.prof[0][i] : A.inflow[0]);
-		vy = (real) (A.prof[1] ? A.prof[1][i] : A.inflow[1]);
-		vz = (real) (A.prof[2] ? A.prof[2][i] : A.inflow[2]);
-	}
-
-	static LBM_DEV void site(const StepArgs& A, int64_t x, int64_t y, int64_t z)
-	{
-		real vx, vy, vz;
-		Nb NX, NY, NZ;
-		site(A, x, y, z, vx, vy, vz, NX, NY, NZ);
-	}
-
-	// one site; returns the pre-collision velocity and the neighbour indices
-	// (the coupled kernel drives the ADE step with them, kernels.h:152-176)
-	static LBM_DEV void site(const StepArgs& A, int64_t x, int64_t y, int64_t z, real& ovx, real& ovy, real& ovz, Nb& NX,
-							 Nb& NY, Nb& NZ)
-	{
-		const int64_t s0 = sidx(A, x + A.ox, y + A.oy, z + A.oz);
-		const int16_t gi = A.map ? A.map[s0] : (int16_t) GEO_PERIODIC;
-		const bool per = gi == GEO_PERIODIC;
-		NX = neighbours<AA>(x, A.X, A.ox, A.dist_x, per);
-		NY = neighbours<AA>(y, A.Y, A.oy, A.dist_y, per);
-		NZ = neighbours<AA>(z, A.Z, A.oz, A.dist_z, per);
-		const real Fx = (real) A.fx, Fy = (real) A.fy, Fz = (real) A.fz;
-		real f[27];
-		real rho = 1, vx = 0, vy = 0, vz = 0;
-		const elem* __restrict__ cur = (const elem*) A.df0;
-
-		if (gi != GEO_NOTHING) {
-			// pull location (bc.h:72-74); OUTFLOW_RIGHT pulls as if located at x-1
-			Nb PX = NX;
-			if (gi == GEO_OUTFLOW_RIGHT)
-				PX.p = PX.o = NX.m;
-			if (gi == GEO_OUTFLOW_RIGHT_INTERP) {
-				// streamingInterpRight (streaming_AB.h:446-478)
-				const real cs = (real) 0.5773502691896257;
-#pragma unroll
-				for (int q = 0; q < 27; q++) {
-					const int64_t yy = NY[-CY[q]], zz = NZ[-CZ[q]];
-					if (CX[q] < 0)
-						f[q] = cs * S::ld(cur + q * A.XYZ + sidx(A, NX.m, yy, zz), fx_sc(q, A.rs)) + ((real) 1 - cs) * S::ld(cur + q * A.XYZ + sidx(A, NX.o, yy, zz), fx_sc(q, A.rs));
-					else
-						f[q] = S::ld(cur + q * A.XYZ + sidx(A, CX[q] > 0 ? NX.m : NX.o, yy, zz), fx_sc(q, A.rs));
-				}
-			}
-			else if (AA && A.even) {
-#pragma unroll
-				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX.o, NY.o, NZ.o), fx_sc(q, A.rs));
-			}
-			else if (AA) {
-#pragma unroll
-				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + opp(q) * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(opp(q), A.rs));
-			}
-			else {
-#pragma unroll
-				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs));
-			}
-
-			const real rest = S::dev ? (real) 1 : (real) 0;
-			switch (gi) {
-				case GEO_INFLOW:
-					inflow(A, y, z, vx, vy, vz);
-					rho = 1;
-					set_eq(f, rho, vx, vy, vz);
-					break;
-				case GEO_INFLOW_LEFT:
-					if constexpr (! S::dev) {
-						// moment-based inflow (bc.h:91-142)
-						inflow(A, y, z, vx, vy, vz);
-						rho = (real) 1 / (1 - vx)
-							* ((f[zzz] + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[zpz] + f[zmz]) + (f[zzp] + f[zzm]))))
-							   + 2 * (f[mzz] + (((f[mpp] + f[mmm]) + (f[mpm] + f[mmp])) + ((f[mpz] + f[mmz]) + (f[mzp] + f[mzm])))));
-						const real n1o3 = (real) (1.0 / 3.0), n1o9 = (real) (1.0 / 9.0);
-						const real m100 = rho * vx, m010 = rho * vy, m001 = rho * vz, m011 = rho * (vy * vz);
-						const real m020 = n1o3 * rho + rho * (vy * vy);
-						const real m002 = n1o3 * rho + rho * (vz * vz);
-						const real m021 = n1o3 * rho * vz + rho * ((vy * vy) * vz);
-						const real m012 = n1o3 * rho * vy + rho * (vy * (vz * vz));
-						const real m022 = n1o9 * rho + n1o3 * rho * (vy * vy + vz * vz) + rho * (vy * vy) * (vz * vz);
-						f[pzz] = m100 + (m022 - (m020 + m002)) + f[mzz]
-							   + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[zzp] + f[zzm]) + (f[zpz] + f[zmz])))
-							   + 2 * (((f[mpp] + f[mmm]) + (f[mpm] + f[mmp])) + ((f[mpz] + f[mmz]) + (f[mzp] + f[mzm])));
-						f[ppz] = (real) 0.5 * ((m020 - m022) + (-m012 + m010)) - (f[mpz] + f[zpz]);
-						f[pmz] = (real) 0.5 * ((m020 - m022) + (m012 - m010)) - (f[mmz] + f[zmz]);
-						f[pzp] = (real) 0.5 * ((m002 - m022) + (-m021 + m001)) - (f[mzp] + f[zzp]);
-						f[pzm] = (real) 0.5 * ((m002 - m022) + (m021 - m001)) - (f[mzm] + f[zzm]);
-						f[ppp] = (real) 0.25 * ((m022 + m011) + (m021 + m012)) - (f[mpp] + f[zpp]);
-						f[ppm] = (real) 0.25 * ((m022 - m011) + (-m021 + m012)) - (f[mpm] + f[zpm]);
-						f[pmp] = (real) 0.25 * ((m022 - m011) + (m021 - m012)) - (f[mmp] + f[zmp]);
-						f[pmm] = (real) 0.25 * ((m022 + m011) + (-m021 - m012)) - (f[mmm] + f[zmm]);
-					}
-					else
-						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
-					break;
-				case GEO_INFLOW_BB_LEFT:
-					{
-						// bc.h:143-165 (pull scheme): reads unstreamed own slots
-						inflow(A, y, z, vx, vy, vz);
-						const real k2 = (real) 2 * (real) 3;
-						const int64_t so = sidx(A, NX.o, NY.o, NZ.o);
-						f[pzz] = S::ld(cur + mzz * A.XYZ + so, fx_sc(mzz, A.rs)) - k2 * (real) (2.0 / 27.0) * rho * (-vx);
-						f[ppz] = S::ld(cur + mmz * A.XYZ + so, fx_sc(mmz, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vy);
-						f[pmz] = S::ld(cur + mpz * A.XYZ + so, fx_sc(mpz, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vy);
-						f[pzp] = S::ld(cur + mzm * A.XYZ + so, fx_sc(mzm, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx - vz);
-						f[pzm] = S::ld(cur + mzp * A.XYZ + so, fx_sc(mzp, A.rs)) - k2 * (real) (1.0 / 54.0) * rho * (-vx + vz);
-						f[ppp] = S::ld(cur + mmm * A.XYZ + so, fx_sc(mmm, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy - vz);
-						f[ppm] = S::ld(cur + mmp * A.XYZ + so, fx_sc(mmp, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx - vy + vz);
-						f[pmp] = S::ld(cur + mpm * A.XYZ + so, fx_sc(mpm, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy - vz);
-						f[pmm] = S::ld(cur + mpp * A.XYZ + so, fx_sc(mpp, A.rs)) - k2 * (real) (1.0 / 216.0) * rho * (-vx + vy + vz);
-						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
-						break;
-					}
-				case GEO_INFLOW_EQ_LEFT:
-					// bc.h:166-185 (deviation storage: the known-direction weights sum to 1)
-					inflow(A, y, z, vx, vy, vz);
-					rho = (real) 1 / (1 - vx)
-						* ((rest + (f[zzz] + (((f[zpp] + f[zmm]) + (f[zpm] + f[zmp])) + ((f[zpz] + f[zmz]) + (f[zzp] + f[zzm])))))
-						   + 2 * (f[mzz] + (((f[mpp] + f[mmm]) + (f[mpm] + f[mmp])) + ((f[mpz] + f[mmz]) + (f[mzp] + f[mzm])))));
-					set_eq(f, rho, vx, vy, vz);
-					break;
-				case GEO_OUTFLOW_EQ:
-					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
-					rho = 1;
-					set_eq(f, rho, vx, vy, vz);
-					break;
-				case GEO_OUTFLOW_RIGHT:
-					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
-					rho = 1;
-					break;
-				case GEO_OUTFLOW_RIGHT_INTERP:
-					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
-#pragma unroll
-					for (int q = 0; q < 27; q++)
-						f[q] += eq_q(q, (real) 1, vx, vy, vz) - eq_q(q, rho, vx, vy, vz);
-					rho = 1;
-					break;
-				case GEO_WALL:
-					// full-way bounce-back (bc.h:201-221)
-					rho = 1;
-					vx = vy = vz = 0;
-#pragma unroll
-					for (int q = 1; q < 27; q += 2) {
-						const real t = f[q];
-						f[q] = f[q + 1];
-						f[q + 1] = t;
-					}
-					break;
-				case GEO_SYM_TOP:
-				case GEO_SYM_BOTTOM:
-				case GEO_SYM_LEFT:
-				case GEO_SYM_RIGHT:
-				case GEO_SYM_BACK:
-				case GEO_SYM_FRONT:
-					{
-						// bc.h:222-293
-						const int axis = (gi == GEO_SYM_TOP || gi == GEO_SYM_BOTTOM) ? 2 : (gi == GEO_SYM_LEFT || gi == GEO_SYM_RIGHT) ? 0 : 1;
-						const int sgn = (gi == GEO_SYM_BOTTOM || gi == GEO_SYM_LEFT || gi == GEO_SYM_BACK) ? 1 : -1;
-#pragma unroll
-						for (int q = 0; q < 27; q++) {
-							const int c = axis == 0 ? CX[q] : axis == 1 ? CY[q] : CZ[q];
-							if (c == sgn) {
-								const int mq = axis == 0 ? qidx(1 - sgn, CY[q] + 1, CZ[q] + 1)
-											 : axis == 1 ? qidx(CX[q] + 1, 1 - sgn, CZ[q] + 1)
-														 : qidx(CX[q] + 1, CY[q] + 1, 1 - sgn);
-								real src = 0;
-#pragma unroll
-								for (int r = 0; r < 27; r++)
-									if (r == mq)
-										src = f[r];
-								f[q] = src;
-							}
-						}
-						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
-						break;
-					}
-				default:
-					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
-					break;
-			}
-			// doCollision (bc.h:406-412)
-			const bool do_coll = gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_RIGHT_INTERP
-							  || gi == GEO_INFLOW_LEFT || gi == GEO_INFLOW_BB_LEFT;
-			if (do_coll)
-				collide<ST, COLL>(f, rho, vx, vy, vz, (real) A.omega1);
-			// postCollision streaming
-			if (AA && A.even) {
-				elem* __restrict__ out = (elem*) A.df0;
-#pragma unroll
-				for (int q = 0; q < 27; q++)
-					S::st(out + opp(q) * A.XYZ + s0, f[q], fx_inv(opp(q), A.invr));
-			}
-			else if (AA) {
-				elem* __restrict__ out = (elem*) A.df0;
-#pragma unroll
-				for (int q = 0; q < 27; q++)
-					S::st(out + q * A.XYZ + sidx(A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q], fx_inv(q, A.invr));
-			}
-			else {
-				elem* __restrict__ out = (elem*) A.df1;
-#pragma unroll
-				for (int q = 0; q < 27; q++)
-					S::st(out + q * A.XYZ + s0, f[q], fx_inv(q, A.invr));
-			}
-		}
-		if (A.out_macro)
-			write_macro(A, s0, rho, vx, vy, vz);
-		ovx = vx;
-		ovy = vy;
-		ovz = vz;
-	}
-};
-
-template <int ST, int COLL, int PATTERN>
-__global__ void __launch_bounds__(256) k_general(const StepArgs A)
-{
-	int64_t x, z, y;
-	if (! block_coords(A, x, z, y))
-		return;
-	General<ST, COLL, PATTERN>::site(A, x, y, z);
-}
-
-// ===========================================================================
-// Coupled NSE + ADE: cudaLBMKernel<NSE,ADE> (kernels.h:114-196)
-// ===========================================================================
-template <int ST, int PATTERN>
-LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename Storage<ST>::real vx,
-					  typename Storage<ST>::real vy, typename Storage<ST>::real vz, const Nb& NX, const Nb& NY, const Nb& NZ)
-{
-	using K = Ade7<ST>;
-	using real = typename K::real;
-	using elem = typename K::elem;
-	constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
-	const int16_t gi = B.map ? B.map[s0] : (int16_t) A_PERIODIC;
-	const real pr = (real) B.phi_ref, sc = (real) B.scale;
-	auto macro_at = [&](const void* p) -> real {
-		return A.macro_f64 ? (real) ((const double*) p)[s0] : (real) ((const float*) p)[s0];
-	};
-	// D3Q7_MACRO_Default::copyQuantities (d3q7/macro.h:38-42)
-	const real D = B.diff ? macro_at(B.diff) : (real) B.D;
-	real phi = 0;
-	if (gi != A_NOTHING) {
-		const elem* cur = (const elem*) B.df0;
-		auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
-		Nb PX = NX;
-		if (gi == A_OUTFLOW_RIGHT)
-			PX.p = PX.o = NX.m;
-		real f[7];
-#pragma unroll
-		for (int q = 0; q < 7; q++) {
-			if (AA && B.even)
-				f[q] = K::ld(cur + q * A.XYZ + s0, q, pr, sc);
-			else if (AA)
-				f[q] = K::ld(cur + opp(q) * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), pr, sc);
-			else
-				f[q] = K::ld(cur + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, pr, sc);
-		}
-		switch (gi) {
-			case A_INFLOW:
-				phi = (real) B.inflow_phi;
-				K::set_eq(f, phi, vx, vy, vz);
-				break;
-			case A_WALL:
-				phi = K::density(f);
-#pragma unroll
-				for (int q = 1; q < 7; q += 2) {
-					const real t = f[q];
-					f[q] = f[q + 1];
-					f[q + 1] = t;
-				}
-				break;
-			case A_SYM_TOP: f[zzm] = f[zzp]; phi = K::density(f); break;
-			case A_SYM_BOTTOM: f[zzp] = f[zzm]; phi = K::density(f); break;
-			case A_SYM_LEFT: f[pzz] = f[mzz]; phi = K::density(f); break;
-			case A_SYM_RIGHT: f[mzz] = f[pzz]; phi = K::density(f); break;
-			case A_SYM_BACK: f[zpz] = f[zmz]; phi = K::density(f); break;
-			case A_SYM_FRONT: f[zmz] = f[zpz]; phi = K::density(f); break;
-			default: phi = K::density(f); break;
-		}
-		if (gi == A_FLUID || gi == A_PERIODIC || gi == A_SOLID || gi == A_OUTFLOW_RIGHT) {
-			K::collide(B.collision, f, phi, vx, vy, vz, D);
-			if (B.source) {
-				const real S = macro_at(B.source);
-#pragma unroll
-				for (int q = 0; q < 7; q++)
-					f[q] += (real) w7(q) * S;
-			}
-		}
-		elem* out = (elem*) (AA ? B.df0 : B.df1);
-#pragma unroll
-		for (int q = 0; q < 7; q++) {
-			if (AA && B.even)
-				K::st(out + opp(q) * A.XYZ + s0, opp(q), f[q], pr, sc);
-			else if (AA)
-				K::st(out + q * A.XYZ + sidx(NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), q, f[q], pr, sc);
-			else
-				K::st(out + q * A.XYZ + s0, q, f[q], pr, sc);
-		}
-	}
-	if (A.out_macro && B.macro) {
-		if (A.macro_f64)
-			((double*) B.macro)[s0] = phi;
-		else
-			((float*) B.macro)[s0] = (float) phi;
-	}
-}
-
-template <int ST, int COLL, int PATTERN>
-__global__ void __launch_bounds__(256) k_nse_ade(const StepArgs A, const AdeArgs B)
-{
-	int64_t x, z, y;
-	if (! block_coords(A, x, z, y))
-		return;
-	typename Storage<ST>::real vx, vy, vz;
-	Nb NX, NY, NZ;
-	General<ST, COLL, PATTERN>::site(A, x, y, z, vx, vy, vz, NX, NY, NZ);
-	const int64_t s0 = (NX.o * A.Zs + NZ.o) * A.Ys + NY.o;
-	ade_site<ST, PATTERN>(A, B, s0, vx, vy, vz, NX, NY, NZ);
-}
-
-template <int ST>
-__global__ void k_ade_set_equilibrium(void* df0, void* df1, int ncopies, int64_t XYZ, int64_t n, const void* phi,
-									  int phi_f64, double phi0, double pr, double sc)
-{
-	using K = Ade7<ST>;
-	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= n)
-		return;
-	const double p = phi ? (phi_f64 ? ((const double*) phi)[s] : (double) ((const float*) phi)[s]) : phi0;
-	typename K::real f[7];
-	K::set_eq(f, (typename K::real) p, 0, 0, 0);
-	for (int k = 0; k < ncopies; k++) {
-		auto* out = (typename K::elem*) (k == 0 ? df0 : df1);
-#pragma unroll
-		for (int q = 0; q < 7; q++)
-			K::st(out + q * XYZ + s, q, f[q], (typename K::real) pr, (typename K::real) sc);
-	}
-}
-
-template <int ST>
-__global__ void k_ade_export(const void* df, double* out, int64_t XYZ, int64_t n, double pr, double sc)
-{
-	using K = Ade7<ST>;
-	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= n)
-		return;
-#pragma unroll
-	for (int q = 0; q < 7; q++)
-		out[q * XYZ + s] = (double) K::ld((const typename K::elem*) df + q * XYZ + s, q, (typename K::real) pr, (typename K::real) sc);
-}
-
-// ===========================================================================
-// Initialisation, transfer and reduction kernels
-// ===========================================================================
-struct FieldArgs
-{
-	void* df[2];
-	int ncopies;
-	int64_t XYZ;
-	int64_t nsites;  // stored sites incl. overlaps
-	const double* rho;
-	const double* vx;
-	const double* vy;
-	const double* vz;
-	double c[4];
-	float rs, invr;
-};
-
-template <int ST, int EQK>
-__global__ void k_set_equilibrium(const FieldArgs A)
-{
-	using S = Storage<ST>;
-	using real = typename S::real;
-	using P = Phys<ST>;
-	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= A.nsites)
-		return;
-	const real rho = (real) (A.rho ? A.rho[s] : A.c[0]);
-	const real vx = (real) (A.vx ? A.vx[s] : A.c[1]);
-	const real vy = (real) (A.vy ? A.vy[s] : A.c[2]);
-	const real vz = (real) (A.vz ? A.vz[s] : A.c[3]);
-	real f[27];
-	General<ST, EQK == 0 ? TNL_LBM_COLL_CUM : TNL_LBM_COLL_MRT_LES, TNL_LBM_PATTERN_AA>::set_eq(f, rho, vx, vy, vz);
-	for (int k = 0; k < A.ncopies; k++) {
-		auto* out = (typename S::elem*) A.df[k];
-#pragma unroll
-		for (int q = 0; q < 27; q++)
-			S::st(out + q * A.XYZ + s, f[q], fx_inv(q, A.invr));
-	}
-}
-
-struct MacroArgs
-{
-	float rs, invr;
-	const void* df;
-	void* macro;
-	int macro_f64;
-	int64_t XYZ;
-	int64_t X, Y, Z, ox, oy, oz, Ys, Zs;
-};
-
-// LBM_BLOCK::computeInitialMacro (lbm_block.hpp:281-305); forces zeroed
-template <int ST>
-__global__ void k_compute_macro(const MacroArgs A)
-{
-	using S = Storage<ST>;
-	using real = typename S::real;
-	const int64_t n = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (n >= A.X * A.Y * A.Z)
-		return;
-	const int64_t y = n % A.Y, t = n / A.Y, z = t % A.Z, x = t / A.Z;
-	const int64_t s = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys + (y + A.oy);
-	real f[27];
-#pragma unroll
-	for (int q = 0; q < 27; q++)
-		f[q] = S::ld((const typename S::elem*) A.df + q * A.XYZ + s, fx_sc(q, A.rs));
-	real rho, vx, vy, vz;
-	Phys<ST>::density_velocity(f, 0, 0, 0, rho, vx, vy, vz);
-	if (A.macro_f64) {
-		double* m = (double*) A.macro;
-		m[s] = rho; m[A.XYZ + s] = vx; m[2 * A.XYZ + s] = vy; m[3 * A.XYZ + s] = vz;
-	}
-	else {
-		float* m = (float*) A.macro;
-		m[s] = rho; m[A.XYZ + s] = vx; m[2 * A.XYZ + s] = vy; m[3 * A.XYZ + s] = vz;
-	}
-}
-
-template <int ST, int Q, bool EXPORT>
-__global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites, float rs, float invr)
-{
-	using S = Storage<ST>;
-	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= nsites)
-		return;
-	auto* p = (typename S::elem*) df;
-#pragma unroll
-	for (int q = 0; q < Q; q++) {
-		const double w = S::dev ? (Q == 27 ? w27(q) : (q == 0 ? 0.25 : 0.125)) : 0.0;
-		if constexpr (EXPORT)
-			buf[q * XYZ + s] = (double) S::ld(p + q * XYZ + s, fx_sc(q, rs)) + w;
-		else
-			S::st(p + q * XYZ + s, (typename S::real) (buf[q * XYZ + s] - w), fx_inv(q, invr));
-	}
-}
-
-// FP64 reductions over the macro field: deterministic two-pass block sums
-constexpr int RED_N = 6;
-constexpr int RED_BLOCKS = 1024;
-
-__device__ void block_sum(double* v, double (*sh)[RED_N])
-{
-	const int t = threadIdx.x;
-#pragma unroll
-	for (int i = 0; i < RED_N; i++)
-		sh[t][i] = v[i];
-	__syncthreads();
-	for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-		if (t < w)
-#pragma unroll
-			for (int i = 0; i < RED_N; i++)
-				sh[t][i] += sh[t + w][i];
-		__syncthreads();
-	}
-}
-
-__global__ void __launch_bounds__(256) k_reduce_macro(const MacroArgs A, double* partial)
-{
-	__shared__ double sh[256][RED_N];
-	double v[RED_N] = {0, 0, 0, 0, 0, 0};
-	const int64_t N = A.X * A.Y * A.Z;
-	for (int64_t n = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (int64_t) gridDim.x * blockDim.x) {
-		const int64_t y = n % A.Y, t = n / A.Y, z = t % A.Z, x = t / A.Z;
-		const int64_t s = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys + (y + A.oy);
-		double rho, vx, vy, vz;
-		if (A.macro_f64) {
-			const double* m = (const double*) A.macro;
-			rho = m[s]; vx = m[A.XYZ + s]; vy = m[2 * A.XYZ + s]; vz = m[3 * A.XYZ + s];
-		}
-		else {
-			const float* m = (const float*) A.macro;
-			rho = m[s]; vx = m[A.XYZ + s]; vy = m[2 * A.XYZ + s]; vz = m[3 * A.XYZ + s];
-		}
-		const double u2 = vx * vx + vy * vy + vz * vz;
-		v[0] += rho;
-		v[1] += rho * u2;
-		v[2] += (rho != rho) ? 1.0 : 0.0;
-		v[3] += (rho - 1.0) * (rho - 1.0);
-		v[4] += u2;
-		v[5] += 1.0;
-	}
-	block_sum(v, sh);
-	if (threadIdx.x == 0)
-#pragma unroll
-		for (int i = 0; i < RED_N; i++)
-			partial[blockIdx.x * RED_N + i] = sh[0][i];
-}
-
-__global__ void __launch_bounds__(256) k_reduce_final(const double* partial, int nparts, double* out)
-{
-	__shared__ double sh[256][RED_N];
-	double v[RED_N] = {0, 0, 0, 0, 0, 0};
-	for (int b = threadIdx.x; b < nparts; b += blockDim.x)
-#pragma unroll
-		for (int i = 0; i < RED_N; i++)
-			v[i] += partial[b * RED_N + i];
-	block_sum(v, sh);
-	if (threadIdx.x == 0)
-#pragma unroll
-		for (int i = 0; i < RED_N; i++)
-			out[i] = sh[0][i];
-}
-
-// ===========================================================================
-// Host side: validation and dispatch
-// ===========================================================================
 int validate(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 {
 	if (! cfg || ! SD)
@@ -935,8 +67,8 @@ int validate(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 		return fail(TNL_LBM_ERR_ARG, "unknown streaming pattern");
 	if (cfg->macro_real != TNL_LBM_REAL_F64 && cfg->macro_real != TNL_LBM_REAL_F32)
 		return fail(TNL_LBM_ERR_ARG, "unknown macro element type");
-	if (cfg->collision < TNL_LBM_COLL_CUM || cfg->collision > TNL_LBM_COLL_CUM_2017_ANTIALIAS)
-		return fail(TNL_LBM_ERR_UNSUPPORTED, "collision operator not built into this library");
+	if (cfg->collision < TNL_LBM_COLL_CUM || cfg->collision > TNL_LBM_COLL_KBC_C4)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "unknown collision operator");
 	if (cfg->storage >= TNL_LBM_STORE_F32_DEV && cfg->collision != TNL_LBM_COLL_CUM)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "deviation storage is built for the default cumulant only");
 	if (! SD->df[0] || (cfg->pattern == TNL_LBM_PATTERN_AB && ! SD->df[1]))
@@ -944,12 +76,6 @@ int validate(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 	return TNL_LBM_OK;
 }
 
-int hip_check(hipError_t e, const char* what)
-{
-	if (e != hipSuccess)
-		return fail(TNL_LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-	return TNL_LBM_OK;
-}
 
 // omega1 = 1/(3 nu + 1/2) in the arithmetic of the storage form (col_cum.h:195)
 double omega1_of(int storage, double nu)
@@ -962,21 +88,6 @@ double omega1_of(int storage, double nu)
 	return (double) (1.0f / d);
 }
 
-template <int ST, int COLL>
-int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
-{
-	if (! A.map && cfg->pattern == TNL_LBM_PATTERN_AA) {
-		if (A.even)
-			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(grid), dim3(threads), 0, st, A);
-		else
-			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(grid), dim3(threads), 0, st, A);
-	}
-	else if (cfg->pattern == TNL_LBM_PATTERN_AA)
-		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, A);
-	else
-		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, A);
-	return hip_check(hipGetLastError(), "step launch");
-}
 
 template <int ST>
 int dispatch_coll(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
@@ -987,7 +98,8 @@ int dispatch_coll(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads
 		switch (cfg->collision) {
 			case TNL_LBM_COLL_CUM: return launch_step<ST, TNL_LBM_COLL_CUM>(cfg, A, threads, grid, st);
 			case TNL_LBM_COLL_CUM_2017: return launch_step<ST, TNL_LBM_COLL_CUM_2017>(cfg, A, threads, grid, st);
-			default: return launch_step<ST, TNL_LBM_COLL_CUM_2017_ANTIALIAS>(cfg, A, threads, grid, st);
+			case TNL_LBM_COLL_CUM_2017_ANTIALIAS: return launch_step<ST, TNL_LBM_COLL_CUM_2017_ANTIALIAS>(cfg, A, threads, grid, st);
+			default: return launch_alt(cfg, A, threads, grid, st);
 		}
 	}
 }
@@ -1126,6 +238,7 @@ static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 	if (SD->lbm_viscosity == 0)
 		return fail(TNL_LBM_ERR_ARG, "LBM viscosity is 0");  // state.hpp:1240-1245
 	A.omega1 = omega1_of(cfg->storage, SD->lbm_viscosity);
+	A.nu = SD->lbm_viscosity;
 	A.fx = SD->fx; A.fy = SD->fy; A.fz = SD->fz;
 	A.inflow[0] = SD->inflow_vx; A.inflow[1] = SD->inflow_vy; A.inflow[2] = SD->inflow_vz;
 	for (int k = 0; k < 3; k++)
@@ -1247,13 +360,17 @@ static int set_eq_impl(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, 
 	F.invr = invr_of(SD);
 	const unsigned grid = (unsigned) ((F.nsites + 255) / 256);
 	hipStream_t st = (hipStream_t) stream;
-	const bool cum = cfg->collision <= TNL_LBM_COLL_CUM_2017_ANTIALIAS;
+	const int eqk = eq_kind_of(cfg->collision);
 	auto go = [&](auto stc) {
 		constexpr int ST = decltype(stc)::value;
-		if (cum)
+		if (eqk == 0)
 			hipLaunchKernelGGL((k_set_equilibrium<ST, 0>), dim3(grid), dim3(256), 0, st, F);
-		else
-			hipLaunchKernelGGL((k_set_equilibrium<ST, 1>), dim3(grid), dim3(256), 0, st, F);
+		else if constexpr (! Storage<ST>::dev) {
+			if (eqk == 1)
+				hipLaunchKernelGGL((k_set_equilibrium<ST, 1>), dim3(grid), dim3(256), 0, st, F);
+			else
+				hipLaunchKernelGGL((k_set_equilibrium<ST, 2>), dim3(grid), dim3(256), 0, st, F);
+		}
 		return hip_check(hipGetLastError(), "set_equilibrium launch");
 	};
 	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));

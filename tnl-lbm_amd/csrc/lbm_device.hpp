@@ -49,6 +49,53 @@ constexpr double w27(int q)
 	return W1[CX[q] + 1] * W1[CY[q] + 1] * W1[CZ[q] + 1];
 }
 
+// Directions in lexicographic order (x slowest; m < z < p on every axis):
+// the summation order of the reference's second-moment and stabiliser sums
+// (col_mrt.h:16-29, col_kbc_n.h:233-252).
+constexpr int LEX27[27] = {mmm, mmz, mmp, mzm, mzz, mzp, mpm, mpz, mpp, zmm, zmz, zmp, zzm, zzz,
+						   zzp, zpm, zpz, zpp, pmm, pmz, pmp, pzm, pzz, pzp, ppm, ppz, ppp};
+// Raw-moment sums of the KBC operators as (direction, sign) sequences in the
+// reference's summation order (col_kbc_n.h:359-382).
+struct MomentTerm
+{
+	int q, s;
+};
+constexpr MomentTerm KB_M200[18] = {{mmm, 1}, {mmp, 1}, {mmz, 1}, {mpm, 1}, {mpp, 1}, {mpz, 1}, {mzm, 1}, {mzp, 1}, {mzz, 1},
+									{pmm, 1}, {pmp, 1}, {pmz, 1}, {ppm, 1}, {ppp, 1}, {ppz, 1}, {pzm, 1}, {pzp, 1}, {pzz, 1}};
+constexpr MomentTerm KB_M020[18] = {{mmm, 1}, {mmp, 1}, {mmz, 1}, {mpm, 1}, {mpp, 1}, {mpz, 1}, {zmm, 1}, {zmp, 1}, {zmz, 1},
+									{zpm, 1}, {zpp, 1}, {zpz, 1}, {pmm, 1}, {pmp, 1}, {pmz, 1}, {ppm, 1}, {ppp, 1}, {ppz, 1}};
+constexpr MomentTerm KB_M002[18] = {{mmm, 1}, {mmp, 1}, {mzm, 1}, {mzp, 1}, {mpm, 1}, {mpp, 1}, {zmm, 1}, {zmp, 1}, {zzm, 1},
+									{zzp, 1}, {zpm, 1}, {zpp, 1}, {pmm, 1}, {pmp, 1}, {pzm, 1}, {pzp, 1}, {ppm, 1}, {ppp, 1}};
+constexpr MomentTerm KB_M110[12] = {{mmm, 1},  {mmz, 1},  {mmp, 1},  {mpm, -1}, {mpz, -1}, {mpp, -1},
+									{pmm, -1}, {pmz, -1}, {pmp, -1}, {ppm, 1},  {ppz, 1},  {ppp, 1}};
+constexpr MomentTerm KB_M101[12] = {{mmm, 1},  {mmp, -1}, {mzm, 1},  {mzp, -1}, {mpm, 1},  {mpp, -1},
+									{pmm, -1}, {pmp, 1},  {pzm, -1}, {pzp, 1},  {ppm, -1}, {ppp, 1}};
+constexpr MomentTerm KB_M011[12] = {{mmm, 1},  {mmp, -1}, {mpm, -1}, {mpp, 1},  {zmm, 1},  {zmp, -1},
+									{zpm, -1}, {zpp, 1},  {pmm, 1},  {pmp, -1}, {ppm, -1}, {ppp, 1}};
+constexpr MomentTerm KB_M111[8] = {{ppp, 1}, {mmp, 1}, {mpm, 1}, {mpp, -1}, {pmm, 1}, {pmp, -1}, {ppm, -1}, {mmm, -1}};
+constexpr MomentTerm KB_M201[12] = {{ppp, 1},  {mmp, 1}, {mzm, -1}, {mzp, 1}, {mpm, -1}, {mpp, 1},
+									{pmm, -1}, {pmp, 1}, {pzm, -1}, {pzp, 1}, {ppm, -1}, {mmm, -1}};
+constexpr MomentTerm KB_M102[12] = {{ppp, 1}, {mmp, -1}, {mzm, -1}, {mzp, -1}, {mpm, -1}, {mpp, -1},
+									{pmm, 1}, {pmp, 1},  {pzm, 1},  {pzp, 1},  {ppm, 1},  {mmm, -1}};
+constexpr MomentTerm KB_M210[12] = {{ppp, 1},  {mmz, -1}, {mmp, -1}, {mpm, 1}, {mpz, 1}, {mpp, 1},
+									{pmm, -1}, {pmz, -1}, {pmp, -1}, {ppm, 1}, {ppz, 1}, {mmm, -1}};
+constexpr MomentTerm KB_M120[12] = {{ppp, 1}, {mmz, -1}, {mmp, -1}, {mpm, -1}, {mpz, -1}, {mpp, -1},
+									{pmm, 1}, {pmz, 1},  {pmp, 1},  {ppm, 1},  {ppz, 1},  {mmm, -1}};
+constexpr MomentTerm KB_M021[12] = {{ppp, 1},  {mmp, 1}, {mpm, -1}, {mpp, 1}, {zmm, -1}, {zmp, 1},
+									{zpm, -1}, {zpp, 1}, {pmm, -1}, {pmp, 1}, {ppm, -1}, {mmm, -1}};
+constexpr MomentTerm KB_M012[12] = {{ppp, 1},  {mmp, -1}, {mpm, 1},  {mpp, 1},  {zmm, -1}, {zmp, -1},
+									{zpm, 1},  {zpp, 1},  {pmm, -1}, {pmp, -1}, {ppm, 1},  {mmm, -1}};
+
+// equilibrium used by a collision operator for initialisation and boundaries
+// (the LBM_EQ default of each reference plugin): 0 = EQ_INV_CUM, 1 = EQ,
+// 2 = EQ_ENTROPIC
+constexpr int eq_kind_of(int coll)
+{
+	return coll <= TNL_LBM_COLL_CUM_2017_ANTIALIAS ? 0
+		 : (coll == TNL_LBM_COLL_MRT_LES || coll == TNL_LBM_COLL_KBC_N1 || coll == TNL_LBM_COLL_KBC_C1) ? 1
+																											: 2;
+}
+
 // ---------------------------------------------------------------------------
 // Storage forms.  `real` is the arithmetic type, `elem` the stored type.
 template <int ST>
@@ -215,13 +262,289 @@ struct Abs
 		const real feq = (real) 1 - (real) 1.5 * (vx * vx + vy * vy + vz * vz) + (real) 3 * cu + (real) 4.5 * cu * cu;
 		return (real) w * rho * feq;
 	}
+	// D3Q27_EQ_ENTROPIC (eq_entropic.h:10-216): rho prod_d W(c_d) (2 - s_d) A_d^c_d
+	// with s_d = sqrt(1 + 3 u_d^2), A_d = (2 u_d + s_d) / (1 - u_d), evaluated
+	// weights first, then the (2 - s_d), then A_d axis by axis
+	template <int q>
+	static LBM_DEV real eq_entropic(real rho, real vx, real vy, real vz)
+	{
+		constexpr int c[3] = {CX[q], CY[q], CZ[q]};
+		const real v[3] = {vx, vy, vz};
+		real r = (real) W1[c[0] + 1] * (real) W1[c[1] + 1] * (real) W1[c[2] + 1];
+#pragma unroll
+		for (int d = 0; d < 3; d++)
+			r = r * ((real) 2 - sqrt((real) 1 + (real) 3 * v[d] * v[d]));
+#pragma unroll
+		for (int d = 0; d < 3; d++) {
+			if (c[d] != 0) {
+				const real A = ((real) 2 * v[d] + sqrt((real) 1 + (real) 3 * v[d] * v[d])) / ((real) 1 - v[d]);
+				r = c[d] < 0 ? r / A : r * A;
+			}
+		}
+		return rho * r;
+	}
+	// KIND 0: EQ_INV_CUM (cumulants), 1: EQ (MRT_LES, KBC N1/C1), 2: EQ_ENTROPIC (other KBC)
 	template <int KIND, int q>
 	static LBM_DEV real eq(real rho, real vx, real vy, real vz)
 	{
 		if constexpr (KIND == 0)
 			return eq_invcum<q>(rho, vx, vy, vz);
-		else
+		else if constexpr (KIND == 1)
 			return eq_poly<q>(rho, vx, vy, vz);
+		else
+			return eq_entropic<q>(rho, vx, vy, vz);
+	}
+
+	// ---- D3Q27_MRT "MRT_LES" (col_mrt.h:7-142) ---------------------------
+	// Second moment Pi (sums over the lexicographic direction order), its
+	// non-equilibrium part relaxed with the Smagorinsky rate
+	//   omega = 2 / (sqrt(tau^2 + 18 C sqrt(Q) / rho) + tau), C = 0.0342,
+	//   Q = 2 |Pi_neq|^2, tau = 3 nu + 1/2,
+	// and f rebuilt from rho, u and Pi with the 2nd-order Hermite form.
+	template <int a, int b>
+	static LBM_DEV real lexsum2(const real* f)
+	{
+		real r = 0;
+		bool first = true;
+#pragma unroll
+		for (int i = 0; i < 27; i++) {
+			const int q = LEX27[i];
+			const int c[3] = {CX[q], CY[q], CZ[q]};
+			const int w = c[a] * c[b];
+			if (w != 0) {
+				r = first ? (w > 0 ? f[q] : -f[q]) : (w > 0 ? r + f[q] : r - f[q]);
+				first = false;
+			}
+		}
+		return r;
+	}
+	static LBM_DEV void mrt(real* f, real rho, real vx, real vy, real vz, real nu)
+	{
+		real P[6] = {lexsum2<0, 0>(f), lexsum2<1, 1>(f), lexsum2<2, 2>(f), lexsum2<0, 1>(f), lexsum2<0, 2>(f), lexsum2<1, 2>(f)};
+		const real third = (real) (1.0 / 3.0);
+		const real Pn[6] = {P[0] - rho * (third + vx * vx), P[1] - rho * (third + vy * vy), P[2] - rho * (third + vz * vz),
+							P[3] - rho * vx * vy,           P[4] - rho * vx * vz,           P[5] - rho * vy * vz};
+		const real Q = (real) 2 * (Pn[0] * Pn[0] + Pn[1] * Pn[1] + Pn[2] * Pn[2] + (real) 2 * (Pn[3] * Pn[3] + Pn[4] * Pn[4] + Pn[5] * Pn[5]));
+		const real tau = (real) 3 * nu + (real) 0.5;
+		const real C = (real) 0.0342;
+		const real omega = (real) 2 / (sqrt(tau * tau + (real) 2 * C * (real) 3 * (real) 3 * sqrt(Q) / rho) + tau);
+#pragma unroll
+		for (int k = 0; k < 6; k++)
+			P[k] -= omega * Pn[k];
+		const real trace = P[0] + P[1] + P[2];
+#pragma unroll
+		for (int q = 0; q < 27; q++) {
+			const real cx = (real) CX[q], cy = (real) CY[q], cz = (real) CZ[q];
+			const int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+			f[q] = (real) w27(q)
+				 * (rho * ((real) 2.5 - (real) 1.5 * (real) nz + (real) 3 * (vx * cx + vy * cy + vz * cz))
+					+ (real) 4.5 * (P[0] * (cx * cx) + P[1] * (cy * cy) + P[2] * (cz * cz) + (real) 2 * (P[3] * (cx * cy) + P[4] * (cx * cz) + P[5] * (cy * cz)))
+					- (real) 1.5 * trace);
+		}
+	}
+
+	// ---- KBC N1-N4 / C1-C4 (col_kbc_n.h:254-1272, col_kbc_c.h:283-1301) ---
+	// f = k + s + h with the shear part s = D (+T) (+Q), in raw (N) or
+	// central (C) moments; beta = 1/(2 nu/cs^2 + 1);
+	//   gamma = 1/beta - (2 - 1/beta) <ds|dh> / <dh|dh>,  <x|y> = sum x y / feq;
+	//   f -= beta (2 ds + gamma dh) - (1 - beta) S feq,  S = 3 (c - u).F / rho;
+	// feq is the product-form equilibrium.  The per-direction quantities are
+	// recomputed in the two passes instead of being held (register pressure).
+	template <int N>
+	static LBM_DEV real msum(const real* f, const MomentTerm (&t)[N])
+	{
+		real r = t[0].s > 0 ? f[t[0].q] : -f[t[0].q];
+#pragma unroll
+		for (int i = 1; i < N; i++)
+			r = t[i].s > 0 ? r + f[t[i].q] : r - f[t[i].q];
+		return r;
+	}
+	// central forms divide (x / 6), raw forms multiply by the reciprocal
+	template <bool DIV>
+	static LBM_DEV real scl(real x, real n, real inv)
+	{
+		if constexpr (DIV)
+			return x / n;
+		else
+			return x * inv;
+	}
+	template <int q, bool DIV>
+	static LBM_DEV real tensD(real Nxz, real Nyz, real Pxy, real Pxz, real Pyz)
+	{
+		constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
+		constexpr int nz = (cx != 0) + (cy != 0) + (cz != 0);
+		const real sixth = (real) (1.0 / 6.0);
+		if constexpr (nz == 1) {
+			if constexpr (cx != 0)
+				return scl<DIV>((real) 2 * Nxz - Nyz, (real) 6, sixth);
+			else if constexpr (cy != 0)
+				return scl<DIV>(-Nxz + (real) 2 * Nyz, (real) 6, sixth);
+			else
+				return scl<DIV>(-Nxz - Nyz, (real) 6, sixth);
+		}
+		else if constexpr (nz == 2) {
+			const real P = cz == 0 ? Pxy : cy == 0 ? Pxz : Pyz;
+			constexpr int sg = cz == 0 ? cx * cy : cy == 0 ? cx * cz : cy * cz;
+			return scl<DIV>(sg > 0 ? P : -P, (real) 4, (real) 0.25);
+		}
+		else
+			return 0;
+	}
+	template <int q, bool DIV>
+	static LBM_DEV real tensT(real T)
+	{
+		constexpr int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+		if constexpr (nz == 0)
+			return -T;
+		else if constexpr (nz == 1)
+			return scl<DIV>(T, (real) 6, (real) (1.0 / 6.0));
+		else
+			return 0;
+	}
+	// Qc: xxy xxz xyy yyz xzz yzz xyz
+	template <int q, bool DIV>
+	static LBM_DEV real tensQ(const real* Qc)
+	{
+		constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
+		constexpr int nz = (cx != 0) + (cy != 0) + (cz != 0);
+		if constexpr (nz == 1) {
+			const real a = cx ? Qc[2] + Qc[4] : cy ? Qc[0] + Qc[5] : Qc[1] + Qc[3];
+			return scl<DIV>((cx + cy + cz) > 0 ? -a : a, (real) 2, (real) 0.5);
+		}
+		else if constexpr (nz == 2) {
+			constexpr int sa = cz == 0 ? cx : cy == 0 ? cx : cy;
+			constexpr int sb = cz == 0 ? cy : cz;
+			const real a = cz == 0 ? Qc[2] : cy == 0 ? Qc[4] : Qc[5];
+			const real b = cz == 0 ? Qc[0] : cy == 0 ? Qc[1] : Qc[3];
+			const real t = sa > 0 ? a : -a;
+			return scl<DIV>(sb > 0 ? t + b : t - b, (real) 4, (real) 0.25);
+		}
+		else if constexpr (nz == 3)
+			return scl<DIV>(cx * cy * cz > 0 ? Qc[6] : -Qc[6], (real) 8, (real) 0.125);
+		else
+			return 0;
+	}
+	struct KbcParts
+	{
+		real X[3][3];  // product-equilibrium factors [axis][c + 1]
+		real T, Nxz, Nyz, Pxy, Pxz, Pyz, Qc[7];
+		real eT, eNxz, eNyz, ePxy, ePxz, ePyz, eQc[7];
+	};
+	template <int q, bool CENTRAL, bool WT, bool WQ>
+	static LBM_DEV void kbc_site(const KbcParts& K, real rho, real fq, real& feq, real& ds, real& dh)
+	{
+		feq = -rho * K.X[0][CX[q] + 1] * K.X[1][CY[q] + 1] * K.X[2][CZ[q] + 1];
+		real s = tensD<q, CENTRAL>(K.Nxz, K.Nyz, K.Pxy, K.Pxz, K.Pyz) - tensD<q, CENTRAL>(K.eNxz, K.eNyz, K.ePxy, K.ePxz, K.ePyz);
+		if constexpr (WT)
+			s = s + tensT<q, CENTRAL>(K.T) - tensT<q, CENTRAL>(K.eT);
+		if constexpr (WQ)
+			s = s + tensQ<q, CENTRAL>(K.Qc) - tensQ<q, CENTRAL>(K.eQc);
+		ds = s;
+		dh = fq - feq - s;
+	}
+	template <bool CENTRAL, bool WT, bool WQ, int i>
+	static LBM_DEV void kbc_gamma_sums(const KbcParts& K, const real* f, real rho, real& num, real& den)
+	{
+		if constexpr (i < 27) {
+			constexpr int q = LEX27[i];
+			real feq, ds, dh;
+			kbc_site<q, CENTRAL, WT, WQ>(K, rho, f[q], feq, ds, dh);
+			const real ifeq = (real) 1 / feq;
+			num = i == 0 ? ds * dh * ifeq : num + ds * dh * ifeq;
+			den = i == 0 ? dh * dh * ifeq : den + dh * dh * ifeq;
+			kbc_gamma_sums<CENTRAL, WT, WQ, i + 1>(K, f, rho, num, den);
+		}
+	}
+	template <bool CENTRAL, bool WT, bool WQ, int q>
+	static LBM_DEV void kbc_update(const KbcParts& K, real* f, real rho, const real* v, const real* F, real beta, real gamma)
+	{
+		if constexpr (q < 27) {
+			real feq, ds, dh;
+			kbc_site<q, CENTRAL, WT, WQ>(K, rho, f[q], feq, ds, dh);
+			constexpr int c[3] = {CX[q], CY[q], CZ[q]};
+			real t[3];
+#pragma unroll
+			for (int d = 0; d < 3; d++)
+				t[d] = c[d] == 0 ? -v[d] * F[d] : (c[d] > 0 ? (-v[d] + (real) 1) * F[d] : (-v[d] - (real) 1) * F[d]);
+			const real S = ((real) 3 * (t[0] + t[1] + t[2])) / rho;
+			f[q] -= beta * ((real) 2 * ds + gamma * dh) - ((real) 1 - beta) * S * feq;
+			kbc_update<CENTRAL, WT, WQ, q + 1>(K, f, rho, v, F, beta, gamma);
+		}
+	}
+	template <bool CENTRAL, bool WT, bool WQ>
+	static LBM_DEV void kbc(real* f, real rho, real vx, real vy, real vz, real nu, real Fx, real Fy, real Fz)
+	{
+		const real v[3] = {vx, vy, vz};
+		const real F[3] = {Fx, Fy, Fz};
+		const real third = (real) (1.0 / 3.0);
+		KbcParts K;
+#pragma unroll
+		for (int d = 0; d < 3; d++) {
+			const real z = third - (real) 1 + v[d] * v[d];
+			const real p = -(real) 0.5 * (z + (real) 1 + v[d]);
+			K.X[d][0] = p + v[d];
+			K.X[d][1] = z;
+			K.X[d][2] = p;
+		}
+		const real M200 = msum(f, KB_M200), M020 = msum(f, KB_M020), M002 = msum(f, KB_M002);
+		const real M110 = msum(f, KB_M110), M101 = msum(f, KB_M101), M011 = msum(f, KB_M011);
+		const real rT = M200 + M020 + M002, rNxz = M200 - M002, rNyz = M020 - M002;
+		if constexpr (! CENTRAL) {
+			K.T = rT; K.Nxz = rNxz; K.Nyz = rNyz; K.Pxy = M110; K.Pxz = M101; K.Pyz = M011;
+			if constexpr (WQ) {
+				K.Qc[0] = msum(f, KB_M210); K.Qc[1] = msum(f, KB_M201); K.Qc[2] = msum(f, KB_M120);
+				K.Qc[3] = msum(f, KB_M021); K.Qc[4] = msum(f, KB_M102); K.Qc[5] = msum(f, KB_M012);
+				K.Qc[6] = msum(f, KB_M111);
+			}
+			K.eT = rho * ((real) 3 * third + vx * vx + vy * vy + vz * vz);
+			K.eNxz = rho * (vx * vx - vz * vz);
+			K.eNyz = rho * (vy * vy - vz * vz);
+			K.ePxy = rho * vx * vy;
+			K.ePxz = rho * vx * vz;
+			K.ePyz = rho * vy * vz;
+			K.eQc[0] = rho * vy * (third + vx * vx);
+			K.eQc[1] = rho * vz * (third + vx * vx);
+			K.eQc[2] = rho * vx * (third + vy * vy);
+			K.eQc[3] = rho * vz * (third + vy * vy);
+			K.eQc[4] = rho * vx * (third + vz * vz);
+			K.eQc[5] = rho * vy * (third + vz * vz);
+			K.eQc[6] = rho * vx * vy * vz;
+		}
+		else {
+			// central macros of col_kbc_c.h:57-69 as written (the cubic ones
+			// carry 3 u^2 and u_x u_y u_z without the factor rho)
+			K.T = rT - rho * (vx * vx + vy * vy + vz * vz);
+			K.Nxz = rNxz + rho * (vz * vz - vx * vx);
+			K.Nyz = rNyz + rho * (vz * vz - vy * vy);
+			K.Pxy = M110 - rho * vx * vy;
+			K.Pxz = M101 - rho * vx * vz;
+			K.Pyz = M011 - rho * vy * vz;
+			if constexpr (WQ) {
+				const real T = K.T, Nxz = K.Nxz, Nyz = K.Nyz, Pxy = K.Pxy, Pxz = K.Pxz, Pyz = K.Pyz;
+				K.Qc[0] = msum(f, KB_M210) - third * ((real) 6 * vx * Pxy + vy * ((real) 3 * vx * vx + (real) 2 * Nxz - Nyz + T));
+				K.Qc[1] = msum(f, KB_M201) - third * ((real) 6 * vx * Pxz + vz * ((real) 3 * vx * vx + (real) 2 * Nxz - Nyz + T));
+				K.Qc[2] = msum(f, KB_M120) - third * ((real) 6 * vy * Pxy + vx * ((real) 3 * vy * vy + (real) 2 * Nyz - Nxz + T));
+				K.Qc[3] = msum(f, KB_M021) - third * ((real) 6 * vy * Pyz + vz * ((real) 3 * vy * vy + (real) 2 * Nyz - Nxz + T));
+				K.Qc[4] = msum(f, KB_M102) - third * ((real) 6 * vz * Pxz + vx * ((real) 3 * vz * vz - Nyz - Nxz + T));
+				K.Qc[5] = msum(f, KB_M012) - third * ((real) 6 * vz * Pyz + vy * ((real) 3 * vz * vz - Nyz - Nxz + T));
+				K.Qc[6] = msum(f, KB_M111) - vx * Pyz - vy * Pxz - vz * Pxy - vx * vy * vz;
+			}
+			K.eT = rho * (real) 3 * third;
+			K.eNxz = K.eNyz = K.ePxy = K.ePxz = K.ePyz = 0;
+#pragma unroll
+			for (int k = 0; k < 7; k++)
+				K.eQc[k] = 0;
+		}
+		if constexpr (! WQ) {
+#pragma unroll
+			for (int k = 0; k < 7; k++)
+				K.Qc[k] = K.eQc[k] = 0;
+		}
+		const real beta = (real) 1 / ((real) 2 * nu / third + (real) 1);
+		real num = 0, den = 0;
+		kbc_gamma_sums<CENTRAL, WT, WQ, 0>(K, f, rho, num, den);
+		const real gamma = (real) 1 / beta - ((real) 2 - (real) 1 / beta) * num / den;
+		kbc_update<CENTRAL, WT, WQ, 0>(K, f, rho, v, F, beta, gamma);
 	}
 
 	// forward 1-D central-moment pass (col_cum.h Eqs 6-14)
