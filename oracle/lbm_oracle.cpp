@@ -32,6 +32,7 @@
 #include <string>
 #include <type_traits>
 #include <algorithm>
+#include <vector>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -1911,6 +1912,190 @@ int import_all(const tnl_lbm_d3q27_data* SD, int which, const double* in)
 }  // namespace
 
 // ===========================================================================
+// ===========================================================================
+// ADJOINT of one A-B step with the default cumulant (the transposed Jacobian
+// of the step map), computed by forward-mode automatic differentiation of
+// the restated site map above: a reference independent of the product's
+// hand-written reverse pass.  Supported geometry codes: FLUID, PERIODIC,
+// WALL, INFLOW, OUTFLOW_EQ, OUTFLOW_RIGHT, NOTHING.
+// ===========================================================================
+namespace adj {
+
+constexpr int ND = 30;  // tangents: 27 pulled populations + 3 inflow velocity components
+struct Dual
+{
+	double v;
+	double d[ND];
+	constexpr Dual() : v(0), d{} {}
+	constexpr Dual(double x) : v(x), d{} {}
+	Dual& operator+=(const Dual& b);
+	Dual& operator-=(const Dual& b);
+	Dual& operator*=(const Dual& b);
+};
+inline Dual operator+(const Dual& a, const Dual& b)
+{
+	Dual r(a.v + b.v);
+	for (int i = 0; i < ND; i++)
+		r.d[i] = a.d[i] + b.d[i];
+	return r;
+}
+inline Dual operator-(const Dual& a, const Dual& b)
+{
+	Dual r(a.v - b.v);
+	for (int i = 0; i < ND; i++)
+		r.d[i] = a.d[i] - b.d[i];
+	return r;
+}
+inline Dual operator-(const Dual& a)
+{
+	Dual r(-a.v);
+	for (int i = 0; i < ND; i++)
+		r.d[i] = -a.d[i];
+	return r;
+}
+inline Dual operator*(const Dual& a, const Dual& b)
+{
+	Dual r(a.v * b.v);
+	for (int i = 0; i < ND; i++)
+		r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+	return r;
+}
+inline Dual operator/(const Dual& a, const Dual& b)
+{
+	Dual r(a.v / b.v);
+	for (int i = 0; i < ND; i++)
+		r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v);
+	return r;
+}
+inline Dual& Dual::operator+=(const Dual& b)
+{
+	return *this = *this + b;
+}
+inline Dual& Dual::operator-=(const Dual& b)
+{
+	return *this = *this - b;
+}
+inline Dual& Dual::operator*=(const Dual& b)
+{
+	return *this = *this * b;
+}
+
+// the A-B site map (pulled populations, inflow velocity) -> post-collision
+// populations for the supported codes, as pre_collision / collide do it
+template <typename T>
+void site_map(int16_t gi, KS27<T>& KS, const T uin[3], double nu, double fx, double fy, double fz)
+{
+	using A = Abs<T>;
+	KS.lbmViscosity = nu;
+	KS.fx = fx;
+	KS.fy = fy;
+	KS.fz = fz;
+	switch (gi) {
+		case GEO_INFLOW:
+			KS.vx = uin[0];
+			KS.vy = uin[1];
+			KS.vz = uin[2];
+			KS.rho = 1;
+			A::setEquilibrium(KS, 0);
+			return;
+		case GEO_OUTFLOW_EQ:
+			A::densityVelocity(KS);
+			KS.rho = 1;
+			A::setEquilibrium(KS, 0);
+			return;
+		case GEO_WALL:
+			for (int q = 1; q < 27; q += 2) {
+				const T t = KS.f[q];
+				KS.f[q] = KS.f[q + 1];
+				KS.f[q + 1] = t;
+			}
+			return;
+		case GEO_OUTFLOW_RIGHT:
+			A::densityVelocity(KS);
+			KS.rho = 1;
+			A::template cumulant<false, false>(KS);
+			return;
+		default:  // FLUID, PERIODIC
+			A::densityVelocity(KS);
+			A::template cumulant<false, false>(KS);
+			return;
+	}
+}
+
+bool supported(int16_t g)
+{
+	return g == GEO_FLUID || g == GEO_PERIODIC || g == GEO_WALL || g == GEO_INFLOW || g == GEO_OUTFLOW_EQ
+		|| g == GEO_OUTFLOW_RIGHT || g == GEO_NOTHING;
+}
+
+template <int STORE>
+int step(const tnl_lbm_d3q27_data* SD, const void* lam_next, void* lam, double* grad)
+{
+	using S = Store<STORE>;
+	using elem = typename S::elem;
+	using N = Site27<STORE, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB>;
+	const Geometry G{SD->X, SD->Y, SD->Z, SD->ox, SD->oy, SD->oz, SD->XYZ};
+	const elem* F = (const elem*) SD->df[0];
+	const elem* Ln = (const elem*) lam_next;
+	elem* L = (elem*) lam;
+	const int64_t NS = SD->X * SD->Y * SD->Z;
+	std::vector<double> acc(27 * (size_t) SD->XYZ, 0.0);
+	const bool dist[3] = {false, false, false};
+	for (int64_t x = 0; x < SD->X; x++)
+		for (int64_t z = 0; z < SD->Z; z++)
+			for (int64_t y = 0; y < SD->Y; y++) {
+				const int64_t s = G.s(x, y, z);
+				const int16_t gi = SD->map ? SD->map[s] : (int16_t) GEO_PERIODIC;
+				if (! supported(gi))
+					return fail(TNL_LBM_ERR_UNSUPPORTED, "geometry code without an adjoint");
+				if (gi == GEO_NOTHING)
+					continue;
+				int64_t xp, xm, yp, ym, zp, zm;
+				N::indices(G, gi == GEO_PERIODIC, dist, x, y, z, xp, xm, yp, ym, zp, zm);
+				int64_t xx = x;
+				if (gi == GEO_OUTFLOW_RIGHT)
+					xp = xx = xm;
+				int64_t src[27];
+				KS27<Dual> KS;
+				for (int q = 0; q < 27; q++) {
+					src[q] = G.s(N::nb(-CX[q], xm, xx, xp), N::nb(-CY[q], ym, y, yp), N::nb(-CZ[q], zm, z, zp));
+					KS.f[q] = Dual((double) S::load(F, q * G.XYZ + src[q], q));
+					KS.f[q].d[q] = 1;
+				}
+				const int64_t iy = SD->Y * z + y;
+				Dual uin[3];
+				uin[0] = Dual(SD->inflow_profile[0] ? SD->inflow_profile[0][iy] : SD->inflow_vx);
+				uin[1] = Dual(SD->inflow_profile[1] ? SD->inflow_profile[1][iy] : SD->inflow_vy);
+				uin[2] = Dual(SD->inflow_profile[2] ? SD->inflow_profile[2][iy] : SD->inflow_vz);
+				for (int d = 0; d < 3; d++)
+					uin[d].d[27 + d] = 1;
+				site_map(gi, KS, uin, SD->lbm_viscosity, SD->fx, SD->fy, SD->fz);
+				double lam_out[27];
+				for (int k = 0; k < 27; k++)
+					lam_out[k] = (double) Ln[k * G.XYZ + s];
+				for (int q = 0; q < 27; q++) {
+					double m = 0;
+					for (int k = 0; k < 27; k++)
+						m += lam_out[k] * KS.f[k].d[q];
+					acc[q * (size_t) G.XYZ + src[q]] += m;
+				}
+				if (grad)
+					for (int d = 0; d < 3; d++) {
+						double g = 0;
+						for (int k = 0; k < 27; k++)
+							g += lam_out[k] * KS.f[k].d[27 + d];
+						grad[d * SD->Y * SD->Z + iy] += g;
+					}
+			}
+	(void) NS;
+	for (int q = 0; q < 27; q++)
+		for (int64_t i = 0; i < G.XYZ; i++)
+			L[q * G.XYZ + i] = (elem) acc[q * (size_t) G.XYZ + i];
+	return TNL_LBM_OK;
+}
+
+}  // namespace adj
+
 extern "C" {
 
 const char* oracle_last_error(void)
@@ -2124,5 +2309,25 @@ uint16_t oracle_f32_to_f16(float v) { return f32_to_f16(v); }
 float oracle_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
 uint16_t oracle_f32_to_bf16(float v) { return f32_to_bf16(v); }
 float oracle_bf16_to_f32(uint16_t h) { return bf16_to_f32(h); }
+
+
+// adjoint of one A-B step: lambda <- J^T lambda_next, grad[3][Y*Z] += d/d(inflow)
+int oracle_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* lambda_next,
+							  void* lambda, double* grad_inflow)
+{
+	if (int rc = check(cfg, SD))
+		return rc;
+	if (cfg->collision != TNL_LBM_COLL_CUM || cfg->pattern != TNL_LBM_PATTERN_AB)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the default cumulant, A-B pattern");
+	if (cfg->storage != TNL_LBM_STORE_F64 && cfg->storage != TNL_LBM_STORE_F32)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the absolute storage forms");
+	if (SD->ox || SD->oy || SD->oz)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for a single block without overlaps");
+	if (! lambda_next || ! lambda)
+		return fail(TNL_LBM_ERR_NULL, "adjoint arrays missing");
+	if (cfg->storage == TNL_LBM_STORE_F64)
+		return adj::step<TNL_LBM_STORE_F64>(SD, lambda_next, lambda, grad_inflow);
+	return adj::step<TNL_LBM_STORE_F32>(SD, lambda_next, lambda, grad_inflow);
+}
 
 }  // extern "C"

@@ -46,6 +46,8 @@ def lib():
         L.oracle_d3q27_collide_dev.argtypes = [C.c_int, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
         L.oracle_d3q27_equilibrium.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.oracle_d3q27_density_velocity.argtypes = [C.c_int, dp, C.c_double, C.c_double, C.c_double, dp]
+        L.oracle_d3q27_adjoint_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_d3q27_adjoint_step.restype = C.c_int
         L.oracle_f32_to_f16.argtypes = [C.c_float]
         L.oracle_f32_to_f16.restype = C.c_uint16
         L.oracle_f16_to_f32.argtypes = [C.c_uint16]
@@ -195,6 +197,25 @@ class HostLattice:
         """rho, vx, vy, vz as [x, y, z] arrays (float64)."""
         m = self.macro.astype(np.float64).reshape(4, self.X, self.Z, self.Y).transpose(0, 1, 3, 2)
         return m[0], m[1], m[2], m[3]
+
+    def adjoint_step(self, lam_next, grad=None, which=None):
+        """Adjoint of one A-B step taken from the state in df[which] (default:
+        the current one): returns J^T lam_next as a (27, X, Y, Z) array and adds
+        d/d(inflow velocity) to grad (shape (3, Z, Y)) when given."""
+        which = self.current_df() if which is None else which
+        el = _NP_ELEM[self.cfg.storage]
+        ln = np.ascontiguousarray(np.asarray(lam_next, dtype=np.float64).transpose(0, 1, 3, 2)).reshape(-1).astype(el)
+        out = np.zeros_like(ln)
+        saved = self.d.df[0]
+        self.d.df[0] = self.df[which].ctypes.data
+        g = None if grad is None else np.ascontiguousarray(grad, dtype=np.float64)
+        try:
+            _chk(lib().oracle_d3q27_adjoint_step(C.byref(self.cfg), C.byref(self.d), _ptr(ln), _ptr(out), _ptr(g)))
+        finally:
+            self.d.df[0] = saved
+        if grad is not None:
+            grad[...] = g
+        return out.astype(np.float64).reshape(27, self.X, self.Z, self.Y).transpose(0, 1, 3, 2)
 
     def step_coupled(self, ade, output_macro=False, flags=0):
         """One coupled NSE + ADE step (kernels.h:114-196): parity and A-B rotation for both lattices."""
