@@ -281,6 +281,25 @@ int tnl_lbm_d3q7_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* 
 int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
                                double out[6], void* stream);
 
+/* ---- adjoint of one step ------------------------------------------------
+ * The transposed Jacobian of one tnl_lbm_d3q27_step with the default
+ * cumulant and the A-B pattern, taken at the state SD->df[0] (the state
+ * that step reads):
+ *   lambda      <- (d df_out / d df[0])^T lambda_next
+ *   grad_inflow += (d df_out / d inflow velocity)^T lambda_next
+ * lambda_next / lambda: 27*XYZ elements of the storage type (F64 or F32),
+ * same layout as the populations; lambda is overwritten, must not alias
+ * lambda_next or df[0].  grad_inflow: 3*Y*Z doubles (component-major, index
+ * Y*z + y as the inflow profiles) accumulated, or NULL.  Geometry codes with
+ * an adjoint: FLUID, PERIODIC, WALL, INFLOW, OUTFLOW_EQ, OUTFLOW_RIGHT,
+ * NOTHING; other codes yield NaN at their sources.  Single block (no
+ * overlaps).  The reference's adjoint machinery is for SRT
+ * (d3q27/col_srt_adjoint.h:4-110, streaming_AB.h:246-281 streamingAdjoint,
+ * bc.h:295-398, driven by sim_adjoint.cu:278-491); this entry point is the
+ * same per-step operation for the default cumulant.                        */
+int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
+                               const void* lambda_next, void* lambda, double* grad_inflow, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,160 @@
+"""GPU parity of the adjoint step (tnl_lbm_d3q27_adjoint_step: the product's
+hand-written reverse pass through the default cumulant) against the oracle's
+forward-mode dual-number Jacobian of the same A-B step, on the same seeded
+state and adjoint input: periodic with a body force, and channels with walls
+at the array edges, an inflow profile and both outflows (including the
+inflow-velocity gradient)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from lbm_common import rel_max
+from pytnl_lbm import D3Q27Lattice, _abi
+from pytnl_lbm._abi import GEO
+
+pytestmark = pytest.mark.gpu
+
+TORCH = {"f64": torch.float64, "f32": torch.float32}
+STORE = {"f64": _abi.STORE_F64, "f32": _abi.STORE_F32}
+
+
+def channel(X, Y, Z, outflow):
+    geo = np.full((X, Y, Z), GEO.FLUID, dtype=np.int16)
+    geo[:, 0, :] = GEO.WALL
+    geo[:, -1, :] = GEO.WALL
+    geo[:, :, 0] = GEO.WALL
+    geo[:, :, -1] = GEO.WALL
+    geo[0, 1:-1, 1:-1] = GEO.INFLOW
+    geo[-1, 1:-1, 1:-1] = outflow
+    return geo
+
+
+def to_dev(a, dtype):
+    """(27, X, Y, Z) -> flat population layout (q, x, z, y) on the GPU."""
+    return torch.as_tensor(np.ascontiguousarray(a.transpose(0, 1, 3, 2)), dtype=dtype).reshape(-1).cuda()
+
+
+def from_dev(t, X, Y, Z):
+    return t.double().cpu().numpy().reshape(27, X, Z, Y).transpose(0, 1, 3, 2)
+
+
+def setup(case, storage, X=6, Y=70, Z=5):
+    geo, force = (None, (1e-4, -2e-4, 5e-5)) if case == "periodic" else (
+        channel(X, Y, Z, GEO.OUTFLOW_RIGHT if case == "outflow_right" else GEO.OUTFLOW_EQ), (0.0, 0.0, 0.0))
+    mreal = _abi.REAL_F64 if storage == "f64" else _abi.REAL_F32
+    g = D3Q27Lattice(X, Y, Z, storage=storage, pattern="AB", nu=0.02, geo=geo, force=force, macro_real=mreal)
+    h = O.HostLattice(X, Y, Z, storage=STORE[storage], pattern=_abi.PATTERN_AB, nu=0.02, geo=geo, force=force,
+                      macro_real=mreal, threads=8)
+    rng = np.random.default_rng(7)
+    prof = [0.03 + 0.01 * rng.standard_normal((Z, Y)), 0.005 * rng.standard_normal((Z, Y)), 0.005 * rng.standard_normal((Z, Y))]
+    for lat in (g, h):
+        lat.set_inflow(0.0, 0.0, 0.0, profile=prof)
+    r = [1 + 0.01 * rng.standard_normal((X, Y, Z))] + list(0.03 * rng.standard_normal((3, X, Y, Z)))
+    h.set_equilibrium_field(*r)
+    f = h.export_f64(0) * (1 + 0.02 * rng.standard_normal((27, X, Y, Z)))
+    h.import_f64(f, which=0)
+    g.import_f64(h.export_f64(0), which=0)  # the stored (possibly rounded) state on both sides
+    w = rng.standard_normal((27, X, Y, Z))
+    return g, h, w
+
+
+@pytest.mark.parametrize("case", ["periodic", "outflow_right", "outflow_eq"])
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_adjoint_step_matches_oracle(case, storage, tol):
+    g, h, w = setup(case, storage)
+    X, Y, Z = g.X, g.Y, g.Z
+    wq = from_dev(to_dev(w, TORCH[storage]), X, Y, Z)  # the adjoint input as stored
+    gh = np.zeros((3, Z, Y))
+    ref = h.adjoint_step(wq, grad=gh, which=0)
+    lam = torch.empty(27 * g.d.XYZ, dtype=TORCH[storage], device="cuda")
+    gg = torch.zeros(3 * Y * Z, dtype=torch.float64, device="cuda")
+    g.adjoint_step(to_dev(wq, TORCH[storage]), lam, gg, which=0)
+    torch.cuda.synchronize()
+    got = from_dev(lam, X, Y, Z)
+    assert np.isfinite(got).all()
+    assert rel_max(got, ref) < tol, (case, storage, rel_max(got, ref))
+    if case != "periodic":
+        assert rel_max(gg.cpu().numpy().reshape(3, Z, Y), gh) < tol
+
+
+def test_adjoint_rejects_unsupported_configurations():
+    g = D3Q27Lattice(4, 4, 4, pattern="AA", nu=0.02)
+    lam = torch.zeros(27 * g.d.XYZ, dtype=torch.float64, device="cuda")
+    with pytest.raises(_abi.LBMError):
+        g.adjoint_step(lam, torch.zeros_like(lam))
+
+
+# ---- the checkpointed, differentiable run against central differences ----------
+
+def duct_problem(X=32, Y=16, Z=16, T=120, xp=24, storage="f64", K=None):
+    from pytnl_lbm import CheckpointedRun, channel_geometry, duct_flow_fields, inflow_control
+    lat = D3Q27Lattice(X, Y, Z, storage=storage, pattern="AB", nu=(0.51 - 0.5) / 3, geo=channel_geometry(X, Y, Z))
+    lat.set_equilibrium_field(*duct_flow_fields(X, Y, Z, 0.04))
+    f0 = lat.df[0].clone()
+    run = CheckpointedRun(lat, T, K)
+    p = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=1), device="cuda")
+    pt = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=2), device="cuda")
+    from pytnl_lbm import density_velocity, lattice_run, populations
+    with torch.no_grad():
+        _, ut = density_velocity(populations(run.forward(pt, f0), lat)[:, xp])
+    ut = ut.clone()
+
+    def J(pp, ff):
+        _, u = density_velocity(populations(lattice_run(run, pp, ff), lat)[:, xp])
+        return 0.5 * ((u - ut) ** 2).sum()
+
+    return lat, run, p, f0, J
+
+
+def test_inflow_gradient_matches_central_differences():
+    """The duct with a perturbed inlet profile, objective 1/2 sum |u - u_target|^2
+    on one x plane after T steps: dJ/d(profile) . d against
+    (J(p + eps d) - J(p - eps d)) / (2 eps), eps = 1e-4, on 6 random directions,
+    within 1e-5 relative; and dJ/d(initial state) on 2 directions."""
+    lat, run, p, f0, J = duct_problem()
+    pr, fr = p.clone().requires_grad_(), f0.clone().requires_grad_()
+    Jv = J(pr, fr)
+    Jv.backward()
+    assert Jv.item() > 0
+    gp, gf = pr.grad.clone(), fr.grad.clone()
+    eps = 1e-4
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    with torch.no_grad():
+        for i in range(6):
+            d = torch.randn(p.shape, generator=gen, device="cuda", dtype=torch.float64)
+            d[:, 0, :] = d[:, -1, :] = 0  # wall sites carry no inflow
+            d[:, :, 0] = d[:, :, -1] = 0
+            d /= d.abs().max()
+            fd = (J(p + eps * d, f0) - J(p - eps * d, f0)).item() / (2 * eps)
+            ad = (gp * d).sum().item()
+            assert abs(ad - fd) <= 1e-5 * abs(fd), (i, ad, fd)
+        for i in range(2):
+            d = torch.randn(f0.shape, generator=gen, device="cuda", dtype=torch.float64) * 1e-2
+            fd = (J(p, f0 + eps * d) - J(p, f0 - eps * d)).item() / (2 * eps)
+            ad = (gf * d).sum().item()
+            assert abs(ad - fd) <= 1e-5 * abs(fd), (i, ad, fd)
+
+
+def test_gradient_does_not_depend_on_the_checkpoint_stride():
+    grads = []
+    for K in (1, None, 120):
+        _, _, p, f0, J = duct_problem(T=120, K=K)
+        pr = p.clone().requires_grad_()
+        J(pr, f0).backward()
+        grads.append(pr.grad)
+    for g in grads[1:]:
+        assert rel_max(g.cpu().numpy(), grads[0].cpu().numpy()) < 1e-13
+
+
+def test_f32_gradient_close_to_f64():
+    """FP32 storage and arithmetic for the forward, the recomputation and the
+    adjoint; FP64 objective and gradient accumulation."""
+    out = []
+    for st in ("f64", "f32"):
+        _, _, p, f0, J = duct_problem(storage=st)
+        pr = p.clone().requires_grad_()
+        J(pr, f0).backward()
+        out.append(pr.grad.cpu().numpy())
+    err = np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0])
+    assert err < 1e-3, err

@@ -1,0 +1,374 @@
+// Adjoint (transposed Jacobian) of one A-B lattice step with the default
+// cumulant collision, for the gradient of an objective with respect to the
+// state and to the inflow velocity.
+//
+// The reference's adjoint (col_srt_adjoint.h, streaming_AB.h:246-281,
+// bc.h:295-398) is for the SRT operator; this one is derived for the default
+// cumulant (col_cum.h:13-511).  Key structure: the post-collision state of
+// the default cumulant depends on the pre-collision populations only through
+// ten raw moments (m0, j, Pi): rho and u, the second-order central moments
+// (the only relaxed cumulants), and the 4th/6th-order products built from
+// them.  So J^T lam = M^T (dG/dM)^T lam with M the constant 10 x 27 moment
+// matrix, and (dG/dM)^T lam is one reverse sweep through the backward
+// central-moment chain (27 columns of 3) and a few dozen scalar products --
+// about the cost of the forward collision, so the adjoint step stays bound
+// by memory traffic.
+#pragma once
+
+#include "lbm_step.hpp"
+
+namespace lbm {
+
+template <typename real>
+struct AdjCum
+{
+	// forward backward-pass of one column, absolute form:
+	//   fz = k0 (1 - v^2) - 2 v k1 - k2,  f-+ = (k0 (v^2 -+ v) + k1 (2 v -+ 1) + k2) / 2
+	static LBM_DEV void bwd(real k0, real k1, real k2, real v, real& fm, real& fz, real& fp)
+	{
+		const real v2 = v * v;
+		fz = k0 * ((real) 1 - v2) - (real) 2 * v * k1 - k2;
+		fm = (k0 * (v2 - v) + k1 * ((real) 2 * v - (real) 1) + k2) * (real) 0.5;
+		fp = (k0 * (v2 + v) + k1 * ((real) 2 * v + (real) 1) + k2) * (real) 0.5;
+	}
+	// its adjoint: (lm, lz, lp) -> (l0, l1, l2), and d/dv accumulated into lv
+	static LBM_DEV void bwd_adj(real k0, real k1, real v, real lm, real lz, real lp, real& l0, real& l1, real& l2, real& lv)
+	{
+		const real S = (real) 0.5 * (lm + lp);
+		const real d = (real) 0.5 * (lp - lm);
+		l2 = S - lz;
+		l1 = (real) 2 * v * l2 + d;
+		l0 = v * (v * l2 + d) + lz;
+		lv += k0 * l1 + (real) 2 * k1 * l2;
+	}
+
+	// mu = (d f_out / d f_in)^T lam for the collision at the pre-collision
+	// populations f.  rho_fixed: the caller has replaced rho by 1 before the
+	// collision (OUTFLOW_RIGHT, bc.h:191-194): the 4th/6th-order products then
+	// divide by 1 while u and the trace keep the actual mass.
+	static LBM_DEV void vjp(const real* f, const real* lam, real Fx, real Fy, real Fz, real omega1, bool rho_fixed, real* mu)
+	{
+		// ---- forward recomputation ---------------------------------------
+		real m0 = 0, jx = 0, jy = 0, jz = 0, Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
+#pragma unroll
+		for (int q = 0; q < 27; q++) {
+			const real v = f[q];
+			m0 += v;
+			if (CX[q]) { jx += CX[q] > 0 ? v : -v; Pxx += v; }
+			if (CY[q]) { jy += CY[q] > 0 ? v : -v; Pyy += v; }
+			if (CZ[q]) { jz += CZ[q] > 0 ? v : -v; Pzz += v; }
+			if (CX[q] * CY[q]) Pxy += CX[q] * CY[q] > 0 ? v : -v;
+			if (CX[q] * CZ[q]) Pxz += CX[q] * CZ[q] > 0 ? v : -v;
+			if (CY[q] * CZ[q]) Pyz += CY[q] * CZ[q] > 0 ? v : -v;
+		}
+		const real hx = (real) 0.5 * Fx, hy = (real) 0.5 * Fy, hz = (real) 0.5 * Fz;
+		const real rv = (real) 1 / m0;
+		const real vx = (jx + hx) * rv, vy = (jy + hy) * rv, vz = (jz + hz) * rv;
+		const real rk = rho_fixed ? (real) 1 : rv;
+		const real k200 = Pxx - vx * ((real) 2 * jx - m0 * vx);
+		const real k020 = Pyy - vy * ((real) 2 * jy - m0 * vy);
+		const real k002 = Pzz - vz * ((real) 2 * jz - m0 * vz);
+		const real k110 = Pxy - vx * jy - vy * (jx - m0 * vx);
+		const real k101 = Pxz - vx * jz - vz * (jx - m0 * vx);
+		const real k011 = Pyz - vy * jz - vz * (jy - m0 * vy);
+		const real om = (real) 1 - omega1;
+		const real third = (real) (1.0 / 3.0);
+		const real E33 = om * (k200 - k020), E34 = om * (k200 - k002), E35 = m0;
+		const real a = third * (E33 + E34 + E35);
+		const real b = third * (-(real) 2 * E33 + E34 + E35);
+		const real c = third * (E33 - (real) 2 * E34 + E35);
+		const real s = om * k110, p = om * k101, qq = om * k011;
+		const real X211 = a * qq + (real) 2 * p * s, X121 = b * p + (real) 2 * s * qq, X112 = c * s + (real) 2 * qq * p;
+		const real X220 = b * a + (real) 2 * s * s, X022 = c * b + (real) 2 * qq * qq, X202 = a * c + (real) 2 * p * p;
+		const real ks211 = X211 * rk, ks121 = X121 * rk, ks112 = X112 * rk;
+		const real ks220 = X220 * rk, ks022 = X022 * rk, ks202 = X202 * rk;
+		const real Y1 = a * ks022 + b * ks202 + c * ks220 + (real) 4 * (qq * ks211 + p * ks121 + s * ks112);
+		const real Y2 = (real) 16 * s * p * qq + (real) 4 * (p * p * b + qq * qq * a + s * s * c) + (real) 2 * a * b * c;
+		const real ks222 = Y1 * rk - Y2 * rk * rk;
+		// x-pass column inputs K[j][k][m] (moment m along x; j, k along y, z)
+		real K[3][3][3];
+		K[0][0][0] = m0;    K[0][0][1] = hx;    K[0][0][2] = a;
+		K[1][0][0] = hy;    K[1][0][1] = s;     K[1][0][2] = 0;
+		K[2][0][0] = b;     K[2][0][1] = 0;     K[2][0][2] = ks220;
+		K[0][1][0] = hz;    K[0][1][1] = p;     K[0][1][2] = 0;
+		K[1][1][0] = qq;    K[1][1][1] = 0;     K[1][1][2] = ks211;
+		K[2][1][0] = 0;     K[2][1][1] = ks121; K[2][1][2] = 0;
+		K[0][2][0] = c;     K[0][2][1] = 0;     K[0][2][2] = ks202;
+		K[1][2][0] = 0;     K[1][2][1] = ks112; K[1][2][2] = 0;
+		K[2][2][0] = ks022; K[2][2][1] = 0;     K[2][2][2] = ks222;
+		real U[3][3][3], T[3][3][3];  // U[i][j][k]: x position i; T[i][j][k]: x, y positions i, j
+#pragma unroll
+		for (int j = 0; j < 3; j++)
+#pragma unroll
+			for (int k = 0; k < 3; k++)
+				bwd(K[j][k][0], K[j][k][1], K[j][k][2], vx, U[0][j][k], U[1][j][k], U[2][j][k]);
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int k = 0; k < 3; k++)
+				bwd(U[i][0][k], U[i][1][k], U[i][2][k], vy, T[i][0][k], T[i][1][k], T[i][2][k]);
+
+		// ---- reverse sweep ---------------------------------------------------
+		real lvx = 0, lvy = 0, lvz = 0;
+		real lT[3][3][3], lU[3][3][3], lK[3][3][3];
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int j = 0; j < 3; j++)
+				bwd_adj(T[i][j][0], T[i][j][1], vz, lam[qidx(i, j, 0)], lam[qidx(i, j, 1)], lam[qidx(i, j, 2)], lT[i][j][0],
+						lT[i][j][1], lT[i][j][2], lvz);
+#pragma unroll
+		for (int i = 0; i < 3; i++)
+#pragma unroll
+			for (int k = 0; k < 3; k++)
+				bwd_adj(U[i][0][k], U[i][1][k], vy, lT[i][0][k], lT[i][1][k], lT[i][2][k], lU[i][0][k], lU[i][1][k], lU[i][2][k], lvy);
+#pragma unroll
+		for (int j = 0; j < 3; j++)
+#pragma unroll
+			for (int k = 0; k < 3; k++)
+				bwd_adj(K[j][k][0], K[j][k][1], vx, lU[0][j][k], lU[1][j][k], lU[2][j][k], lK[j][k][0], lK[j][k][1], lK[j][k][2], lvx);
+		// (the first-order inputs hx, hy, hz are constants: post-collision
+		// k_100 = -(j_x - m0 u_x) = F_x / 2 identically)
+		real lm0 = lK[0][0][0];
+		real la = lK[0][0][2], lb = lK[2][0][0], lc = lK[0][2][0];
+		real ls = lK[1][0][1], lp = lK[0][1][1], lqq = lK[1][1][0];
+		real lks220 = lK[2][0][2], lks211 = lK[1][1][2], lks121 = lK[2][1][1], lks202 = lK[0][2][2], lks112 = lK[1][2][1];
+		real lks022 = lK[2][2][0];
+		const real lks222 = lK[2][2][2];
+		real lrk = 0;
+		// ks222 = Y1 rk - Y2 rk^2
+		const real lY1 = lks222 * rk, lY2 = -lks222 * rk * rk;
+		lrk += lks222 * (Y1 - (real) 2 * Y2 * rk);
+		la += lY1 * ks022; lks022 += lY1 * a;
+		lb += lY1 * ks202; lks202 += lY1 * b;
+		lc += lY1 * ks220; lks220 += lY1 * c;
+		lqq += (real) 4 * lY1 * ks211; lks211 += (real) 4 * lY1 * qq;
+		lp += (real) 4 * lY1 * ks121;  lks121 += (real) 4 * lY1 * p;
+		ls += (real) 4 * lY1 * ks112;  lks112 += (real) 4 * lY1 * s;
+		ls += lY2 * ((real) 16 * p * qq + (real) 8 * s * c);
+		lp += lY2 * ((real) 16 * s * qq + (real) 8 * p * b);
+		lqq += lY2 * ((real) 16 * s * p + (real) 8 * qq * a);
+		lb += lY2 * ((real) 4 * p * p + (real) 2 * a * c);
+		la += lY2 * ((real) 4 * qq * qq + (real) 2 * b * c);
+		lc += lY2 * ((real) 4 * s * s + (real) 2 * a * b);
+		// ks_xyz = X_xyz rk
+		lrk += lks211 * X211 + lks121 * X121 + lks112 * X112 + lks220 * X220 + lks022 * X022 + lks202 * X202;
+		const real l211 = lks211 * rk, l121 = lks121 * rk, l112 = lks112 * rk;
+		const real l220 = lks220 * rk, l022 = lks022 * rk, l202 = lks202 * rk;
+		la += l211 * qq; lqq += l211 * a; lp += (real) 2 * l211 * s; ls += (real) 2 * l211 * p;
+		lb += l121 * p; lp += l121 * b; ls += (real) 2 * l121 * qq; lqq += (real) 2 * l121 * s;
+		lc += l112 * s; ls += l112 * c; lqq += (real) 2 * l112 * p; lp += (real) 2 * l112 * qq;
+		lb += l220 * a; la += l220 * b; ls += (real) 4 * l220 * s;
+		lc += l022 * b; lb += l022 * c; lqq += (real) 4 * l022 * qq;
+		la += l202 * c; lc += l202 * a; lp += (real) 4 * l202 * p;
+		// a, b, c <- E33, E34, E35 = m0
+		const real lE33 = third * (la - (real) 2 * lb + lc);
+		const real lE34 = third * (la + lb - (real) 2 * lc);
+		lm0 += third * (la + lb + lc);
+		const real lk200 = om * (lE33 + lE34), lk020 = -om * lE33, lk002 = -om * lE34;
+		const real lk110 = om * ls, lk101 = om * lp, lk011 = om * lqq;
+		if (! rho_fixed)
+			lm0 -= lrk * rk * rk;
+		// central moments <- raw moments, velocity, mass
+		real ljx = 0, ljy = 0, ljz = 0;
+		lvx += lk200 * ((real) 2 * (m0 * vx - jx)); ljx -= (real) 2 * vx * lk200; lm0 += vx * vx * lk200;
+		lvy += lk020 * ((real) 2 * (m0 * vy - jy)); ljy -= (real) 2 * vy * lk020; lm0 += vy * vy * lk020;
+		lvz += lk002 * ((real) 2 * (m0 * vz - jz)); ljz -= (real) 2 * vz * lk002; lm0 += vz * vz * lk002;
+		lvx += lk110 * (m0 * vy - jy); lvy += lk110 * (m0 * vx - jx); ljy -= vx * lk110; ljx -= vy * lk110; lm0 += vx * vy * lk110;
+		lvx += lk101 * (m0 * vz - jz); lvz += lk101 * (m0 * vx - jx); ljz -= vx * lk101; ljx -= vz * lk101; lm0 += vx * vz * lk101;
+		lvy += lk011 * (m0 * vz - jz); lvz += lk011 * (m0 * vy - jy); ljz -= vy * lk011; ljy -= vz * lk011; lm0 += vy * vz * lk011;
+		// u = (j + F/2) / m0
+		ljx += lvx * rv; ljy += lvy * rv; ljz += lvz * rv;
+		lm0 -= (lvx * vx + lvy * vy + lvz * vz) * rv;
+		// raw moments are linear in f
+#pragma unroll
+		for (int q = 0; q < 27; q++) {
+			const real cx = (real) CX[q], cy = (real) CY[q], cz = (real) CZ[q];
+			mu[q] = lm0 + cx * ljx + cy * ljy + cz * ljz + (cx * cx) * lk200 + (cy * cy) * lk020 + (cz * cz) * lk002
+				  + (cx * cy) * lk110 + (cx * cz) * lk101 + (cy * cz) * lk011;
+		}
+	}
+
+	// gradient with respect to u of sum_q lam_q feq_q(1, u), feq = D3Q27_EQ_INV_CUM
+	template <int c>
+	static LBM_DEV real phi(real v)
+	{
+		if constexpr (c == 0)
+			return (real) 3 * v * v - (real) 2;
+		else if constexpr (c > 0)
+			return (real) 3 * v * v + (real) 3 * v + (real) 1;
+		else
+			return (real) 3 * v * v - (real) 3 * v + (real) 1;
+	}
+	template <int c>
+	static LBM_DEV real dphi(real v)
+	{
+		return (real) 6 * v + (real) (3 * c);
+	}
+	template <int q>
+	static LBM_DEV void eq_grad_q(const real* lam, real vx, real vy, real vz, real& gx, real& gy, real& gz)
+	{
+		constexpr int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
+		constexpr double coef = nz == 0 ? -1.0 / 27.0 : nz == 1 ? 1.0 / 54.0 : nz == 2 ? -1.0 / 108.0 : 1.0 / 216.0;
+		const real w = (real) coef * lam[q];
+		const real px = phi<CX[q]>(vx), py = phi<CY[q]>(vy), pz = phi<CZ[q]>(vz);
+		gx += w * dphi<CX[q]>(vx) * py * pz;
+		gy += w * px * dphi<CY[q]>(vy) * pz;
+		gz += w * px * py * dphi<CZ[q]>(vz);
+	}
+	template <int q = 0>
+	static LBM_DEV void eq_grad(const real* lam, real vx, real vy, real vz, real& gx, real& gy, real& gz)
+	{
+		if constexpr (q < 27) {
+			eq_grad_q<q>(lam, vx, vy, vz, gx, gy, gz);
+			eq_grad<q + 1>(lam, vx, vy, vz, gx, gy, gz);
+		}
+	}
+};
+
+// adjoint arrays of one step
+struct AdjPtrs
+{
+	const void* lam_next;  // adjoint of the state the forward step wrote (27 XYZ)
+	void* lam;             // adjoint of the state the forward step read (27 XYZ, output)
+	double* grad;          // d/d(inflow velocity), 3 x (Y Z) indexed Y z + y, accumulated; or null
+};
+
+// One thread per site.  Pass SPECIAL = false writes, for every site and q, the
+// site's contribution to the slot it would pull from under periodic wrapping
+// (that map is a bijection, so every output slot is written exactly once:
+// plain stores, no clearing); slots that the site really pulls from somewhere
+// else (array-edge clamps of non-periodic sites, OUTFLOW_RIGHT's pull from
+// x-1, NOTHING sites that pull nothing) get 0 there.  Pass SPECIAL = true then
+// adds those contributions to the slots actually pulled, atomically.
+template <int ST, bool SPECIAL>
+__global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjPtrs P)
+{
+	using S = Storage<ST>;
+	using real = typename S::real;
+	using elem = typename S::elem;
+	using D = AdjCum<real>;
+	int64_t x, z, y;
+	if (! block_coords(A, x, z, y))
+		return;
+	const int64_t s0 = (x * A.Zs + z) * A.Ys + y;
+	const int16_t gi = A.map ? A.map[s0] : (int16_t) GEO_PERIODIC;
+	const bool per = gi == GEO_PERIODIC;
+	const Nb NX = neighbours<false>(x, A.X, 0, false, per);
+	const Nb NY = neighbours<false>(y, A.Y, 0, false, per);
+	const Nb NZ = neighbours<false>(z, A.Z, 0, false, per);
+	Nb PX = NX;
+	if (gi == GEO_OUTFLOW_RIGHT)
+		PX.p = PX.o = NX.m;
+	const Nb WX = neighbours<false>(x, A.X, 0, false, true);
+	const Nb WY = neighbours<false>(y, A.Y, 0, false, true);
+	const Nb WZ = neighbours<false>(z, A.Z, 0, false, true);
+	auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
+	uint32_t irregular = 0;
+#pragma unroll
+	for (int q = 0; q < 27; q++) {
+		const bool same = PX[-CX[q]] == WX[-CX[q]] && NY[-CY[q]] == WY[-CY[q]] && NZ[-CZ[q]] == WZ[-CZ[q]];
+		if (gi == GEO_NOTHING || ! same)
+			irregular |= 1u << q;
+	}
+	if (SPECIAL && (gi == GEO_NOTHING || irregular == 0))
+		return;
+
+	real mu[27];
+	if (gi == GEO_NOTHING) {
+#pragma unroll
+		for (int q = 0; q < 27; q++)
+			mu[q] = 0;
+	}
+	else {
+		const elem* F = (const elem*) A.df0;
+		const elem* Ln = (const elem*) P.lam_next;
+		real f[27], lam[27];
+#pragma unroll
+		for (int q = 0; q < 27; q++) {
+			lam[q] = S::ld(Ln + q * A.XYZ + s0);
+			f[q] = S::ld(F + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+		}
+		switch (gi) {
+			case GEO_FLUID:
+			case GEO_PERIODIC:
+				D::vjp(f, lam, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, false, mu);
+				break;
+			case GEO_OUTFLOW_RIGHT:
+				D::vjp(f, lam, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, true, mu);
+				break;
+			case GEO_WALL:
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					mu[q] = lam[opp(q)];
+				break;
+			case GEO_INFLOW:
+				{
+#pragma unroll
+					for (int q = 0; q < 27; q++)
+						mu[q] = 0;
+					if (! SPECIAL && P.grad) {
+						const int64_t iy = A.Y * z + y;
+						const real ux = (real) (A.prof[0] ? A.prof[0][iy] : A.inflow[0]);
+						const real uy = (real) (A.prof[1] ? A.prof[1][iy] : A.inflow[1]);
+						const real uz = (real) (A.prof[2] ? A.prof[2][iy] : A.inflow[2]);
+						real gx = 0, gy = 0, gz = 0;
+						D::eq_grad(lam, ux, uy, uz, gx, gy, gz);
+						const int64_t n = A.Y * A.Z;
+						atomicAdd(P.grad + iy, (double) gx);
+						atomicAdd(P.grad + n + iy, (double) gy);
+						atomicAdd(P.grad + 2 * n + iy, (double) gz);
+					}
+					break;
+				}
+			case GEO_OUTFLOW_EQ:
+				{
+					// f_out = feq(1, u(f_in)), u = (j + F/2) / m0
+					real m0 = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+					for (int q = 0; q < 27; q++) {
+						m0 += f[q];
+						jx += (real) CX[q] * f[q];
+						jy += (real) CY[q] * f[q];
+						jz += (real) CZ[q] * f[q];
+					}
+					const real rv = (real) 1 / m0;
+					const real ux = (jx + (real) 0.5 * (real) A.fx) * rv, uy = (jy + (real) 0.5 * (real) A.fy) * rv,
+							   uz = (jz + (real) 0.5 * (real) A.fz) * rv;
+					real gx = 0, gy = 0, gz = 0;
+					D::eq_grad(lam, ux, uy, uz, gx, gy, gz);
+#pragma unroll
+					for (int q = 0; q < 27; q++)
+						mu[q] = (gx * ((real) CX[q] - ux) + gy * ((real) CY[q] - uy) + gz * ((real) CZ[q] - uz)) * rv;
+					break;
+				}
+			default:
+				// no adjoint for this geometry code: make the result unusable, visibly
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					mu[q] = (real) __builtin_nan("");
+				break;
+		}
+	}
+
+	elem* L = (elem*) P.lam;
+	if (! SPECIAL) {
+#pragma unroll
+		for (int q = 0; q < 27; q++) {
+			const real v = (irregular >> q) & 1u ? (real) 0 : mu[q];
+			S::st(L + q * A.XYZ + sidx(WX[-CX[q]], WY[-CY[q]], WZ[-CZ[q]]), v);
+		}
+	}
+	else {
+#pragma unroll
+		for (int q = 0; q < 27; q++)
+			if ((irregular >> q) & 1u)
+				atomicAdd(L + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), (elem) mu[q]);
+	}
+}
+
+// both passes of one adjoint step (built in lbm_adjoint.hip)
+int launch_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdjPtrs& P, unsigned threads, unsigned grid,
+				   hipStream_t st);
+
+}  // namespace lbm

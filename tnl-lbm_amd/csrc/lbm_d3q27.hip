@@ -20,6 +20,7 @@
 #include <string>
 #include <type_traits>
 
+#include "lbm_adjoint.hpp"
 #include "lbm_step.hpp"
 
 using namespace lbm;
@@ -468,3 +469,32 @@ int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
 }
 
 }  // extern "C"
+
+int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* lambda_next,
+							   void* lambda, double* grad_inflow, void* stream)
+{
+	if (! cfg || ! SD)
+		return fail(TNL_LBM_ERR_NULL, "null config or data");
+	if (cfg->collision != TNL_LBM_COLL_CUM || cfg->pattern != TNL_LBM_PATTERN_AB)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the default cumulant with the A-B pattern");
+	if (cfg->storage != TNL_LBM_STORE_F64 && cfg->storage != TNL_LBM_STORE_F32)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the absolute storage forms (F64, F32)");
+	if (SD->ox || SD->oy || SD->oz)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for a single block without overlaps");
+	if (! lambda_next || ! lambda)
+		return fail(TNL_LBM_ERR_NULL, "adjoint arrays missing");
+	if (lambda == lambda_next || lambda == SD->df[0])
+		return fail(TNL_LBM_ERR_ARG, "lambda must not alias lambda_next or the state");
+	tnl_lbm_d3q27_data D = *SD;  // only df[0] is read; the pull step needs no second array
+	if (! D.df[1])
+		D.df[1] = D.df[0];
+	StepArgs A;
+	unsigned threads = 0, grid = 0;
+	const int64_t off[3] = {0, 0, 0}, end[3] = {SD->X, SD->Y, SD->Z};
+	if (int rc = make_step_args(cfg, &D, off, end, 0, A, threads, grid))
+		return rc;
+	if (grid == 0)
+		return TNL_LBM_OK;
+	const AdjPtrs P{lambda_next, lambda, grad_inflow};
+	return launch_adjoint(cfg, A, P, threads, grid, (hipStream_t) stream);
+}

@@ -1,0 +1,157 @@
+"""Differentiable lattice runs: T A-B steps behind torch.autograd.Function,
+with checkpointed recomputation over time.
+
+The reference's adjoint workflow (sim_adjoint/sim_adjoint.cu:595-777) runs a
+primary simulation with an inflow profile, then an adjoint simulation whose
+kernel (col_srt_adjoint.h, streaming_AB.h, bc.h) accumulates the gradient of
+a velocity-mismatch loss with respect to that profile.  Here the adjoint is the
+exact discrete adjoint of the product's own step (tnl_lbm_d3q27_adjoint_step:
+default cumulant, A-B pattern), so a loss written in torch over the final
+populations differentiates back to the inflow profile and the initial state.
+
+Memory: sqrt(T) checkpoints of the state are kept during the forward run; the
+backward pass recomputes one segment of about sqrt(T) states from its
+checkpoint, then runs the adjoint steps over it in reverse.  The recomputed
+states are written in place (each step's output buffer is the next state's
+storage), so neither the checkpoints nor the segment cost an extra copy.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import torch
+
+from . import _abi
+from .lattice import D3Q27Lattice, _ptr, _stream_ptr
+
+# D3Q27 velocity set in the reference's direction order (defs.h:317-350)
+D3Q27_CX = (0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1)
+D3Q27_CY = (0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1)
+D3Q27_CZ = (0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1, 1, -1, -1, 1)
+
+
+def populations(f: torch.Tensor, lat: D3Q27Lattice) -> torch.Tensor:
+    """Flat storage-layout populations -> (27, X, Z, Y) view."""
+    return f.view(27, lat.d.XYZ)[:, :lat.nsites].view(27, lat.X, lat.Z, lat.Y)
+
+
+def density_velocity(f: torch.Tensor, force=(0.0, 0.0, 0.0)):
+    """rho and u = (j + F/2)/rho of populations shaped (27, ...), in FP64 and
+    differentiable (D3Q27_COMMON::computeDensityAndVelocity, common.h:17-50,
+    with the half-force shift of col_cum.h:20-22)."""
+    f = f.double()
+    c = torch.tensor([D3Q27_CX, D3Q27_CY, D3Q27_CZ], dtype=torch.float64, device=f.device)
+    rho = f.sum(0)
+    j = torch.tensordot(c, f, dims=([1], [0]))
+    F = torch.tensor(force, dtype=torch.float64, device=f.device).view(3, *([1] * (f.dim() - 1)))
+    return rho, (j + 0.5 * F) / rho
+
+
+class CheckpointedRun:
+    """T A-B steps of a lattice from a given initial state, with a checkpoint
+    every K steps (default K = ceil(sqrt(T))).
+
+    The inflow velocity used by every step is the lattice's inflow profile
+    (NSE_Data_InflowProfile), which ``forward`` sets from a (3, Z, Y) float64
+    control tensor.  Storage F64 or F32; collision CUM."""
+
+    def __init__(self, lat: D3Q27Lattice, T: int, K: int | None = None):
+        if lat.cfg.pattern != _abi.PATTERN_AB:
+            raise ValueError("the adjoint differentiates the A-B step: build the lattice with pattern='AB'")
+        if lat.cfg.collision != _abi.COLL_CUM or lat.cfg.storage not in (_abi.STORE_F64, _abi.STORE_F32):
+            raise ValueError("the adjoint supports the default cumulant with F64 or F32 storage")
+        if T < 1:
+            raise ValueError("T must be at least 1")
+        self.lat, self.T = lat, T
+        self.K = K or max(1, math.ceil(math.sqrt(T)))
+        self.nseg = math.ceil(T / self.K)
+        el = lat.df[0].dtype
+        n = lat.df[0].numel()
+        dev = lat.device
+        # every buffer starts zeroed so that slots a step does not write hold the same values everywhere
+        self.ckpt = [torch.zeros(n, dtype=el, device=dev) for _ in range(self.nseg)]
+        self.pp = [torch.zeros(n, dtype=el, device=dev) for _ in range(2)]
+        self.seg = [torch.zeros(n, dtype=el, device=dev) for _ in range(self.K - 1)]
+        self.lam = [torch.zeros(n, dtype=el, device=dev) for _ in range(2)]
+        self.profile = torch.zeros(3, lat.Z, lat.Y, dtype=torch.float64, device=dev)
+        lat.set_inflow(0.0, 0.0, 0.0, profile=[self.profile[0], self.profile[1], self.profile[2]])
+        # set_inflow copies non-contiguous input; bind the control buffer itself
+        for k in range(3):
+            lat._profiles[k] = self.profile[k]
+            lat.d.inflow_profile[k] = self.profile[k].data_ptr()
+
+    def _step(self, src: torch.Tensor, dst: torch.Tensor, stream=None):
+        lat = self.lat
+        lat.d.even_iter = 1
+        lat.d.df[0], lat.d.df[1] = src.data_ptr(), dst.data_ptr()
+        lat._check(lat.lib.tnl_lbm_d3q27_step(C.byref(lat.cfg), C.byref(lat.d), _abi.I64x3(0, 0, 0),
+                                              _abi.I64x3(lat.X, lat.Y, lat.Z), 0, _stream_ptr(stream)))
+
+    def _adjoint(self, state: torch.Tensor, lam_next: torch.Tensor, lam: torch.Tensor, grad: torch.Tensor,
+                 stream=None):
+        lat = self.lat
+        lat.d.df[0], lat.d.df[1] = state.data_ptr(), None
+        lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step(C.byref(lat.cfg), C.byref(lat.d), _ptr(lam_next), _ptr(lam),
+                                                      _ptr(grad), _stream_ptr(stream)))
+
+    def forward(self, inflow: torch.Tensor, f0: torch.Tensor) -> torch.Tensor:
+        """Runs T steps from f0 (flat storage layout) with the given inflow
+        profile; returns the buffer holding the final state (valid until the
+        next call)."""
+        try:
+            self.profile.copy_(inflow.reshape(3, self.lat.Z, self.lat.Y))
+            self.ckpt[0].copy_(f0.reshape(-1))
+            src = self.ckpt[0]
+            for t in range(self.T):
+                nxt = t + 1
+                if nxt % self.K == 0 and nxt < self.T:
+                    dst = self.ckpt[nxt // self.K]
+                else:
+                    dst = self.pp[1] if src is self.pp[0] else self.pp[0]
+                self._step(src, dst)
+                src = dst
+            return src
+        finally:
+            self.lat._bind()
+
+    def backward(self, lam_T: torch.Tensor, grad: torch.Tensor) -> torch.Tensor:
+        """lam_T = dL/d(final state); adds dL/d(inflow) to grad (3, Z, Y) float64
+        and returns dL/d(initial state) (a buffer valid until the next call)."""
+        try:
+            cur, other = self.lam
+            cur.copy_(lam_T.reshape(-1))
+            for j in reversed(range(self.nseg)):
+                t0 = j * self.K
+                m = min(self.T, t0 + self.K) - t0
+                states = [self.ckpt[j]] + self.seg[:m - 1]
+                for i in range(m - 1):
+                    self._step(states[i], states[i + 1])
+                for i in reversed(range(m)):
+                    self._adjoint(states[i], cur, other, grad)
+                    cur, other = other, cur
+            return cur
+        finally:
+            self.lat._bind()
+
+
+class _LatticeRun(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, inflow, f0, run: CheckpointedRun):
+        fT = run.forward(inflow.detach(), f0.detach())
+        ctx.run = run
+        return fT.clone()
+
+    @staticmethod
+    def backward(ctx, gF):
+        run = ctx.run
+        grad = torch.zeros(3, run.lat.Z, run.lat.Y, dtype=torch.float64, device=gF.device)
+        lam0 = run.backward(gF.to(run.lam[0].dtype).contiguous(), grad)
+        return grad, lam0.clone(), None
+
+
+def lattice_run(run: CheckpointedRun, inflow: torch.Tensor, f0: torch.Tensor) -> torch.Tensor:
+    """Differentiable T-step run: (inflow profile (3, Z, Y) float64, initial
+    populations (flat storage layout)) -> final populations (same layout).
+    Gradients flow to both inputs through the hand-written adjoint."""
+    return _LatticeRun.apply(inflow, f0, run)

@@ -121,3 +121,63 @@ def turbulence_fields_torch(N: int, seed: int, k0: float, u_rms: float, device="
     rms = torch.sqrt((u[0] ** 2 + u[1] ** 2 + u[2] ** 2).mean() / 3.0)
     s = u_rms / rms
     return [c * s for c in u]
+
+
+def channel_geometry(X: int, Y: int, Z: int, outflow: int | None = None) -> np.ndarray:
+    """Square duct along x: bounce-back WALL sites on the y and z edges, the
+    equilibrium velocity inlet (INFLOW) on the x = 0 plane and the
+    zero-gradient outlet (OUTFLOW_RIGHT, bc.h:355-371) on x = X-1, as the
+    reference's channel setups place them (sim_adjoint.cu:51-66 and
+    setBoundaryX/Y/Z in lbm_block.hpp).  Returns an int16 [x, y, z] array of
+    GEO codes."""
+    from ._abi import GEO
+    geo = np.full((X, Y, Z), GEO.FLUID, dtype=np.int16)
+    geo[0, :, :] = GEO.INFLOW
+    geo[-1, :, :] = GEO.OUTFLOW_RIGHT if outflow is None else outflow
+    geo[:, 0, :] = GEO.WALL
+    geo[:, -1, :] = GEO.WALL
+    geo[:, :, 0] = GEO.WALL
+    geo[:, :, -1] = GEO.WALL
+    return geo
+
+
+def duct_profile(Y: int, Z: int, umax: float) -> np.ndarray:
+    """Parabolic x-velocity of the duct, umax at the centre and zero on the wall
+    sites y = 0, Y-1 and z = 0, Z-1; shape (Z, Y)."""
+    y = np.arange(Y, dtype=np.float64) / (Y - 1)
+    z = np.arange(Z, dtype=np.float64) / (Z - 1)
+    return umax * 16.0 * np.outer(z * (1 - z), y * (1 - y))
+
+
+def smooth_perturbation(Y: int, Z: int, amplitude: float, seed: int, modes: int = 4) -> np.ndarray:
+    """Seeded smooth random field of shape (3, Z, Y): for each velocity
+    component a sum of sin(m pi y') sin(n pi z') modes (m, n <= modes,
+    coefficients N(0,1)/(m^2+n^2)) that vanishes on the walls, scaled to
+    max |.| = amplitude."""
+    rng = np.random.default_rng(seed)
+    y = np.arange(Y, dtype=np.float64) / (Y - 1)
+    z = np.arange(Z, dtype=np.float64) / (Z - 1)
+    out = np.zeros((3, Z, Y))
+    for k in range(3):
+        for m in range(1, modes + 1):
+            for n in range(1, modes + 1):
+                a = rng.standard_normal() / (m * m + n * n)
+                out[k] += a * np.outer(np.sin(n * np.pi * z), np.sin(m * np.pi * y))
+        out[k] *= amplitude / np.abs(out[k]).max()
+    return out
+
+
+def inflow_control(Y: int, Z: int, umax: float, amplitude: float, seed: int) -> np.ndarray:
+    """Inlet velocity profile (3, Z, Y): the duct profile in x plus a seeded
+    smooth perturbation in every component."""
+    p = smooth_perturbation(Y, Z, amplitude, seed)
+    p[0] += duct_profile(Y, Z, umax)
+    return p
+
+
+def duct_flow_fields(X: int, Y: int, Z: int, umax: float):
+    """Developed duct flow as initial condition: rho = 1, u = the parabolic
+    duct profile on every x plane.  Returns [x, y, z] arrays (rho, vx, vy, vz)."""
+    vx = np.broadcast_to(duct_profile(Y, Z, umax).T[None], (X, Y, Z)).copy()
+    zero = np.zeros((X, Y, Z))
+    return np.ones((X, Y, Z)), vx, zero, zero.copy()

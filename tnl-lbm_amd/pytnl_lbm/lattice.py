@@ -169,6 +169,27 @@ class D3Q27Lattice:
     def current_df(self):
         return self.iterations % 2 if self.cfg.pattern == _abi.PATTERN_AB else 0
 
+    # ---- adjoint ----------------------------------------------------------------
+    def adjoint_step(self, lam_next: torch.Tensor, lam: torch.Tensor, grad_inflow: torch.Tensor | None = None,
+                     which: int | None = None, stream=None):
+        """lam <- J^T lam_next for one A-B step taken from df[which] (default: the
+        current state); grad_inflow (3, Z, Y) float64 accumulates the gradient
+        with respect to the inflow velocity.  lam_next / lam are flat 27*XYZ
+        tensors of the storage dtype in the population layout."""
+        which = self.current_df() if which is None else which
+        el = _TORCH_ELEM[self.cfg.storage]
+        for t in (lam_next, lam):
+            assert t.dtype == el and t.numel() == 27 * self.d.XYZ and t.is_contiguous()
+        if grad_inflow is not None:
+            assert grad_inflow.dtype == torch.float64 and grad_inflow.numel() == 3 * self.Y * self.Z
+        saved = self.d.df[0], self.d.df[1]
+        self.d.df[0] = self.df[which].data_ptr()
+        try:
+            self._check(self.lib.tnl_lbm_d3q27_adjoint_step(C.byref(self.cfg), C.byref(self.d), _ptr(lam_next), _ptr(lam),
+                                                            _ptr(grad_inflow), _stream_ptr(stream)))
+        finally:
+            self.d.df[0], self.d.df[1] = saved
+
     # ---- readback -------------------------------------------------------------
     def export_f64(self, which=None) -> torch.Tensor:
         """Absolute populations, slot by slot, as a (27, X, Y, Z) float64 view."""
