@@ -39,6 +39,9 @@ def parse():
     p.add_argument("--err-steps", type=int, default=500)
     p.add_argument("--no-precision", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--adjoint-steps", type=int, default=2000)
+    p.add_argument("--adjoint-fd-dirs", type=int, default=16)
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--storage", default="f64", choices=["f64", "f32", "f32dev", "f16dev", "bf16dev", "i16dev"],
                    help="storage form of the headline case (f64 is the headline workload; the others exist "
@@ -139,6 +142,92 @@ def precision_errors(N, steps, storages):
         res[st] = e
     return res, {"tgv": f"{N}^3", "U0": 0.05, "tau": 0.505, "steps": steps, "i16_store_range": 0.5,
                  "rho_norm": "relative to rho-1 of the FP64 run"}
+
+
+def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=20):
+    """Duct 256x96x96 (walls in y, z; equilibrium inlet at x = 0, zero-gradient
+    outlet at x = X-1), tau = 0.51, developed-flow initial state.  Control: the
+    inlet profile (parabolic, Umax = 0.04, plus a seeded smooth perturbation of
+    amplitude 0.005); objective J = 1/2 sum |u - u_target|^2 in FP64 on the
+    x = xp plane after T steps, the target from a second seed.  Times the
+    forward A-B step alone and the forward + checkpointed backward through
+    torch.autograd, and checks dJ/dp against central differences."""
+    import torch
+    from pytnl_lbm import (CheckpointedRun, D3Q27Lattice, _abi, channel_geometry, density_velocity,
+                           duct_flow_fields, inflow_control, lattice_run, populations)
+
+    lat = D3Q27Lattice(X, Y, Z, storage=storage, pattern="AB", nu=(0.51 - 0.5) / 3, geo=channel_geometry(X, Y, Z))
+    lat.set_equilibrium_field(*duct_flow_fields(X, Y, Z, 0.04))
+    f0 = lat.df[0].clone()
+    run = CheckpointedRun(lat, T)
+    p = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=1), device="cuda")
+    pt = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=2), device="cuda")
+    with torch.no_grad():
+        _, ut = density_velocity(populations(run.forward(pt, f0), lat)[:, xp])
+        ut = ut.clone()
+
+    def J(pp):
+        _, u = density_velocity(populations(lattice_run(run, pp, f0), lat)[:, xp])
+        return 0.5 * ((u - ut) ** 2).sum()
+
+    sites = X * Y * Z
+    # forward alone: T A-B steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.forward(p, f0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    # forward + backward
+    pr = p.clone().requires_grad_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Jv = J(pr)
+    Jv.backward()
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t0
+    g = pr.grad.clone()
+    # adjoint kernel alone, on one state
+    lam = [torch.randn(f0.numel(), device="cuda", dtype=f0.dtype) * 1e-3 for _ in range(2)]
+    grad = torch.zeros(3, Z, Y, dtype=torch.float64, device="cuda")
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(adj_reps)]
+    for i in range(adj_reps + 2):
+        if i >= 2:
+            ev[i - 2][0].record()
+        run._adjoint(run.ckpt[0], lam[i % 2], lam[1 - i % 2], grad)
+        if i >= 2:
+            ev[i - 2][1].record()
+    torch.cuda.synchronize()
+    lat._bind()
+    adj_s = sum(a.elapsed_time(b) for a, b in ev) / adj_reps * 1e-3
+    el = _abi.STORAGE_BYTES[lat.storage]
+    adj_bytes = 3 * 27 * el * sites
+    res = {
+        "workload": f"duct {X}x{Y}x{Z}, tau=0.51, T={T}, J on plane x={xp}, {storage.upper()} A-B cumulant",
+        "forward_mlups": round(sites * T / t_fwd / 1e6, 1),
+        "fwd_bwd_mlups": round(sites * T / t_fb / 1e6, 1),
+        "ratio": round(t_fwd / t_fb, 4),
+        "checkpoint_stride": run.K,
+        "checkpoints": run.nseg,
+        "J": Jv.item(),
+        "adjoint_kernel": {"ms_avg": round(adj_s * 1e3, 3), "bytes_per_site": 3 * 27 * el,
+                           "achieved_GBs": round(adj_bytes / adj_s / 1e9, 1),
+                           "roofline_frac": round(adj_bytes / adj_s / 1e9 / PEAK_HBM_GBS, 4)},
+    }
+    if fd_dirs:
+        gen = torch.Generator(device="cuda").manual_seed(11)
+        worst = 0.0
+        with torch.no_grad():
+            for _ in range(fd_dirs):
+                d = torch.randn(p.shape, generator=gen, device="cuda", dtype=torch.float64)
+                d[:, 0, :] = d[:, -1, :] = 0
+                d[:, :, 0] = d[:, :, -1] = 0
+                d /= d.abs().max()
+                fd = (J(p + eps * d) - J(p - eps * d)).item() / (2 * eps)
+                worst = max(worst, abs((g * d).sum().item() - fd) / abs(fd))
+        res["fd_check"] = {"eps": eps, "directions": fd_dirs, "max_rel_err": float(f"{worst:.3e}")}
+    del run, lat
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_baseline(seconds):
@@ -261,6 +350,10 @@ def main():
             per[f"{pst}_{P}"]["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in e.items()}
         out["per_precision"] = per
         out["precision_error_case"] = spec
+    if not args.no_adjoint:
+        out["adjoint"] = {st: adjoint_case(256, 96, 96, args.adjoint_steps, 200, storage=st,
+                                           fd_dirs=args.adjoint_fd_dirs if st == "f64" else 0)
+                          for st in ("f64", "f32")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -158,3 +158,20 @@ def test_f32_gradient_close_to_f64():
         out.append(pr.grad.cpu().numpy())
     err = np.linalg.norm(out[1] - out[0]) / np.linalg.norm(out[0])
     assert err < 1e-3, err
+
+
+def test_outflow_right_inside_the_box_is_poisoned():
+    """The fix-up pass covers the faces of the box; an OUTFLOW_RIGHT site
+    elsewhere has no exact adjoint here, so its slots must come out NaN
+    rather than silently wrong, and every other slot stays finite."""
+    X, Y, Z = 8, 6, 5
+    geo = np.full((X, Y, Z), GEO.PERIODIC, dtype=np.int16)
+    geo[4, 2, 2] = GEO.OUTFLOW_RIGHT
+    g = D3Q27Lattice(X, Y, Z, pattern="AB", nu=0.02, geo=geo)
+    g.set_equilibrium(1.0, 0.01, 0.0, 0.0)
+    lam_next = torch.ones(27 * g.d.XYZ, dtype=torch.float64, device="cuda")
+    lam = torch.zeros_like(lam_next)
+    g.adjoint_step(lam_next, lam, which=0)
+    torch.cuda.synchronize()
+    bad = torch.isnan(lam).sum().item()
+    assert 0 < bad <= 27

@@ -8,12 +8,16 @@ namespace lbm {
 namespace {
 
 template <int ST>
-int adjoint_for(const StepArgs& A, const AdjPtrs& P, unsigned threads, unsigned grid, hipStream_t st)
+int adjoint_for(const StepArgs& A0, const AdjPtrs& P, unsigned threads, unsigned grid, hipStream_t st)
 {
+	StepArgs A = A0;
+	linearize(A, threads, grid);
 	hipLaunchKernelGGL((k_adjoint_ab<ST, false>), dim3(grid), dim3(threads), 0, st, A, P);
 	if (int rc = hip_check(hipGetLastError(), "adjoint launch"))
 		return rc;
-	hipLaunchKernelGGL((k_adjoint_ab<ST, true>), dim3(grid), dim3(threads), 0, st, A, P);
+	const uint64_t shell = shell_count(A.X, A.Y, A.Z);
+	const unsigned sgrid = (unsigned) ((shell + 255) / 256);
+	hipLaunchKernelGGL((k_adjoint_ab<ST, true>), dim3(sgrid), dim3(256), 0, st, A, P);
 	return hip_check(hipGetLastError(), "adjoint fix-up launch");
 }
 

@@ -241,6 +241,55 @@ struct AdjPtrs
 // else (array-edge clamps of non-periodic sites, OUTFLOW_RIGHT's pull from
 // x-1, NOTHING sites that pull nothing) get 0 there.  Pass SPECIAL = true then
 // adds those contributions to the slots actually pulled, atomically.
+// The faces of the box [0,X)x[0,Y)x[0,Z), each site once: x faces whole,
+// then the y faces of the interior x, then the z faces of the interior x and
+// y.  Only face sites can pull from a source other than the periodic wrap
+// (non-periodic neighbour indices clamp at the array edge), and OUTFLOW_RIGHT
+// belongs on the x = X-1 face.
+inline uint64_t shell_count(int64_t X, int64_t Y, int64_t Z)
+{
+	const uint64_t fx = X > 1 ? 2 : 1, fy = Y > 1 ? 2 : 1, fz = Z > 1 ? 2 : 1;
+	const uint64_t ix = X > 2 ? X - 2 : 0, iy = Y > 2 ? Y - 2 : 0;
+	return fx * Y * Z + ix * fy * Z + ix * iy * fz;
+}
+
+LBM_DEV bool shell_coords(const StepArgs& A, int64_t& x, int64_t& z, int64_t& y)
+{
+	uint64_t i = (uint64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	const uint64_t X = A.X, Y = A.Y, Z = A.Z;
+	const uint64_t fx = X > 1 ? 2 : 1, fy = Y > 1 ? 2 : 1, fz = Z > 1 ? 2 : 1;
+	const uint64_t ix = X > 2 ? X - 2 : 0, iy = Y > 2 ? Y - 2 : 0;
+	const uint64_t n1 = fx * Y * Z, n2 = ix * fy * Z, n3 = ix * iy * fz;
+	if (i < n1) {
+		const uint64_t r = i % (Y * Z);
+		x = i / (Y * Z) ? (int64_t) X - 1 : 0;
+		z = (int64_t) (r / Y);
+		y = (int64_t) (r % Y);
+		return true;
+	}
+	i -= n1;
+	if (i < n2) {
+		const uint64_t r = i % (fy * Z);
+		x = 1 + (int64_t) (i / (fy * Z));
+		z = (int64_t) (r / fy);
+		y = r % fy ? (int64_t) Y - 1 : 0;
+		return true;
+	}
+	i -= n2;
+	if (i < n3) {
+		const uint64_t r = i % (iy * fz);
+		x = 1 + (int64_t) (i / (iy * fz));
+		y = 1 + (int64_t) (r / fz);
+		z = r % fz ? (int64_t) Z - 1 : 0;
+		return true;
+	}
+	return false;
+}
+
+// SPECIAL = false: every site, plain stores to the slots it owns under the
+// periodic wrap (0 for slots whose true reader is elsewhere).  SPECIAL = true:
+// the face sites only, atomic adds of those contributions to their true
+// sources, after the first pass.
 template <int ST, bool SPECIAL>
 __global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjPtrs P)
 {
@@ -249,7 +298,7 @@ __global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjP
 	using elem = typename S::elem;
 	using D = AdjCum<real>;
 	int64_t x, z, y;
-	if (! block_coords(A, x, z, y))
+	if (SPECIAL ? ! shell_coords(A, x, z, y) : ! block_coords(A, x, z, y))
 		return;
 	const int64_t s0 = (x * A.Zs + z) * A.Ys + y;
 	const int16_t gi = A.map ? A.map[s0] : (int16_t) GEO_PERIODIC;
@@ -273,12 +322,16 @@ __global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjP
 	}
 	if (SPECIAL && (gi == GEO_NOTHING || irregular == 0))
 		return;
+	const bool on_shell = x == 0 || x == A.X - 1 || y == 0 || y == A.Y - 1 || z == 0 || z == A.Z - 1;
+	// an irregular pull away from the faces (OUTFLOW_RIGHT inside the box) has no fix-up: poison it
+	const bool unsupported = gi != GEO_NOTHING && irregular != 0 && ! on_shell;
 
 	real mu[27];
-	if (gi == GEO_NOTHING) {
+	if (gi == GEO_NOTHING || unsupported) {
+		const real v = unsupported ? (real) __builtin_nan("") : (real) 0;
 #pragma unroll
 		for (int q = 0; q < 27; q++)
-			mu[q] = 0;
+			mu[q] = v;
 	}
 	else {
 		const elem* F = (const elem*) A.df0;
@@ -355,7 +408,7 @@ __global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjP
 	if (! SPECIAL) {
 #pragma unroll
 		for (int q = 0; q < 27; q++) {
-			const real v = (irregular >> q) & 1u ? (real) 0 : mu[q];
+			const real v = (irregular >> q) & 1u && ! unsupported ? (real) 0 : mu[q];
 			S::st(L + q * A.XYZ + sidx(WX[-CX[q]], WY[-CY[q]], WZ[-CZ[q]]), v);
 		}
 	}

@@ -296,6 +296,7 @@ int tnl_lbm_nse_ade_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* ns
 	B.collision = ade->collision;
 	B.even = ade->even_iter ? 1 : 0;
 	hipStream_t st = (hipStream_t) stream;
+	linearize(A, threads, grid);
 	auto go = [&](auto stc) {
 		constexpr int ST = decltype(stc)::value;
 		if (cfg->pattern == TNL_LBM_PATTERN_AA)

@@ -37,9 +37,10 @@ struct StepArgs
 	float cls_sc[4], cls_inv[4];
 	int32_t offx, offy, offz;
 	int32_t nx, ny, nz;
-	uint32_t nyb;      // y segments per row
+	uint32_t nyb;      // y segments per row (linear mapping: blocks per x plane)
 	uint32_t nblocks;  // logical blocks
 	uint32_t per_xcd;  // padded blocks / 8
+	int32_t lin;       // linear y-z mapping (see linearize)
 	int32_t even;
 	int32_t out_macro;
 	int32_t macro_f64;
@@ -54,12 +55,25 @@ struct StepArgs
 constexpr int NXCD = 8;
 
 // logical block -> (x, z, y segment) with XCD-aware remapping
+// LIN = false: whole rows only (x and z stay uniform per block, as the
+// periodic kernel's scalar addressing needs)
+template <bool LIN = true>
 LBM_DEV bool block_coords(const StepArgs& A, int64_t& x, int64_t& z, int64_t& y)
 {
 	const uint32_t b = blockIdx.x;
 	const uint32_t L = (b % NXCD) * A.per_xcd + b / NXCD;
 	if (L >= A.nblocks)
 		return false;
+	if (LIN && A.lin) {
+		// consecutive lanes walk the y-z plane of one x, row after row
+		const uint32_t i = (L % A.nyb) * blockDim.x + threadIdx.x;
+		x = A.offx + (int64_t) (L / A.nyb);
+		if (i >= (uint32_t) A.ny * (uint32_t) A.nz)
+			return false;
+		z = A.offz + (int64_t) (i / (uint32_t) A.ny);
+		y = A.offy + (int64_t) (i % (uint32_t) A.ny);
+		return true;
+	}
 	const uint32_t yb = L % A.nyb;
 	const uint32_t t = L / A.nyb;
 	z = A.offz + (int64_t) (t % (uint32_t) A.nz);
@@ -227,7 +241,7 @@ k_aa_periodic(const StepArgs A)
 	constexpr bool PIN = (LBM_PIN_ADDR >> ST) & 1;
 	constexpr bool PIN_ODD = (LBM_PIN_ODD >> ST) & 1;
 	int64_t x, z, y;
-	if (! block_coords(A, x, z, y))
+	if (! block_coords<false>(A, x, z, y))
 		return;
 	char* __restrict__ df = (char*) A.df0;
 	const uint64_t pb = A.plane_bytes;
@@ -910,6 +924,24 @@ static __global__ void __launch_bounds__(256) k_reduce_final(const double* parti
 // ===========================================================================
 // Host side: validation and dispatch
 // ===========================================================================
+// Kernels that compute every lane's coordinates (k_general, k_nse_ade, the
+// adjoint) can take the y-z plane of one x as one linear range, so that a row
+// length that is not a multiple of the block (Y = 96, say) still fills every
+// wave.  The map-free periodic kernel keeps whole rows per block.
+inline void linearize(StepArgs& A, unsigned& threads, unsigned& grid)
+{
+	if (A.nx == 0 || A.ny % (int32_t) threads == 0)
+		return;
+	threads = 256;
+	const uint64_t plane = (uint64_t) A.ny * (uint64_t) A.nz;
+	A.nyb = (uint32_t) ((plane + threads - 1) / threads);
+	A.nblocks = A.nyb * (uint32_t) A.nx;  // no larger than the row mapping's count
+	const uint32_t padded = (A.nblocks + NXCD - 1) / NXCD * NXCD;
+	A.per_xcd = padded / NXCD;
+	A.lin = 1;
+	grid = padded;
+}
+
 template <int ST, int COLL>
 int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
 {
@@ -919,10 +951,14 @@ int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, 
 		else
 			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(grid), dim3(threads), 0, st, A);
 	}
-	else if (cfg->pattern == TNL_LBM_PATTERN_AA)
-		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, A);
-	else
-		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, A);
+	else {
+		StepArgs B = A;
+		linearize(B, threads, grid);
+		if (cfg->pattern == TNL_LBM_PATTERN_AA)
+			hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, B);
+		else
+			hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, B);
+	}
 	return hip_check(hipGetLastError(), "step launch");
 }
 
