@@ -144,7 +144,7 @@ def precision_errors(N, steps, storages):
                  "rho_norm": "relative to rho-1 of the FP64 run"}
 
 
-def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=20):
+def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=20, mode="segment"):
     """Duct 256x96x96 (walls in y, z; equilibrium inlet at x = 0, zero-gradient
     outlet at x = X-1), tau = 0.51, developed-flow initial state.  Control: the
     inlet profile (parabolic, Umax = 0.04, plus a seeded smooth perturbation of
@@ -159,7 +159,7 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
     lat = D3Q27Lattice(X, Y, Z, storage=storage, pattern="AB", nu=(0.51 - 0.5) / 3, geo=channel_geometry(X, Y, Z))
     lat.set_equilibrium_field(*duct_flow_fields(X, Y, Z, 0.04))
     f0 = lat.df[0].clone()
-    run = CheckpointedRun(lat, T)
+    run = CheckpointedRun(lat, T, mode=mode)
     p = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=1), device="cuda")
     pt = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=2), device="cuda")
     with torch.no_grad():
@@ -171,12 +171,18 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
         return 0.5 * ((u - ut) ** 2).sum()
 
     sites = X * Y * Z
-    # forward alone: T A-B steps
+    # the plain forward: T A-B steps, no checkpoints, no recording
+    run.profile.copy_(p)
+    src, dst = run.pp
+    src.copy_(f0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run.forward(p, f0)
+    for _ in range(T):
+        run._step(src, dst)
+        src, dst = dst, src
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
+    lat._bind()
     # forward + backward
     pr = p.clone().requires_grad_()
     torch.cuda.synchronize()
@@ -186,30 +192,36 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
     torch.cuda.synchronize()
     t_fb = time.perf_counter() - t0
     g = pr.grad.clone()
-    # adjoint kernel alone, on one state
+    # adjoint kernel alone (the form this schedule uses), on one state or recording
     lam = [torch.randn(f0.numel(), device="cuda", dtype=f0.dtype) * 1e-3 for _ in range(2)]
     grad = torch.zeros(3, Z, Y, dtype=torch.float64, device="cuda")
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(adj_reps)]
+    tape = None if run.tape is None else run.tape[0]
     for i in range(adj_reps + 2):
         if i >= 2:
             ev[i - 2][0].record()
-        run._adjoint(run.ckpt[0], lam[i % 2], lam[1 - i % 2], grad)
+        run._adjoint(run.ckpt[0] if tape is None else None, lam[i % 2], lam[1 - i % 2], grad, tape)
         if i >= 2:
             ev[i - 2][1].record()
     torch.cuda.synchronize()
     lat._bind()
     adj_s = sum(a.elapsed_time(b) for a, b in ev) / adj_reps * 1e-3
     el = _abi.STORAGE_BYTES[lat.storage]
-    adj_bytes = 3 * 27 * el * sites
+    # lambda_next read + lambda written + the Jacobian's input (27 pulled populations or 10 recorded moments)
+    jac = 27 * el if tape is None else 10 * tape.element_size()
+    adj_site = 2 * 27 * el + jac
+    adj_bytes = adj_site * sites
     res = {
         "workload": f"duct {X}x{Y}x{Z}, tau=0.51, T={T}, J on plane x={xp}, {storage.upper()} A-B cumulant",
+        "schedule": mode if tape is None else f"{mode} ({'float64' if run.tape_f64 else 'float32'} moment recording)",
         "forward_mlups": round(sites * T / t_fwd / 1e6, 1),
         "fwd_bwd_mlups": round(sites * T / t_fb / 1e6, 1),
         "ratio": round(t_fwd / t_fb, 4),
-        "checkpoint_stride": run.K,
-        "checkpoints": run.nseg,
+        "ratio_definition": "fwd_bwd_mlups / forward_mlups; forward = the plain A-B step, T steps",
+        "checkpoint_stride": run.K if mode != "tape" else None,
+        "checkpoints": run.nseg if mode != "tape" else 0,
         "J": Jv.item(),
-        "adjoint_kernel": {"ms_avg": round(adj_s * 1e3, 3), "bytes_per_site": 3 * 27 * el,
+        "adjoint_kernel": {"ms_avg": round(adj_s * 1e3, 3), "bytes_per_site": adj_site,
                            "achieved_GBs": round(adj_bytes / adj_s / 1e9, 1),
                            "roofline_frac": round(adj_bytes / adj_s / 1e9 / PEAK_HBM_GBS, 4)},
     }
@@ -351,9 +363,12 @@ def main():
         out["per_precision"] = per
         out["precision_error_case"] = spec
     if not args.no_adjoint:
-        out["adjoint"] = {st: adjoint_case(256, 96, 96, args.adjoint_steps, 200, storage=st,
-                                           fd_dirs=args.adjoint_fd_dirs if st == "f64" else 0)
-                          for st in ("f64", "f32")}
+        adj = {}
+        for st, mode in (("f64", "segment"), ("f64", "tape"), ("f64", "recompute"), ("f32", "segment"),
+                         ("f32", "tape")):
+            adj[f"{st}_{mode}"] = adjoint_case(256, 96, 96, args.adjoint_steps, 200, storage=st, mode=mode,
+                                               fd_dirs=args.adjoint_fd_dirs if st == "f64" and mode != "recompute" else 0)
+        out["adjoint"] = adj
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

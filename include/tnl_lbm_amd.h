@@ -292,13 +292,37 @@ int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
  * lambda_next or df[0].  grad_inflow: 3*Y*Z doubles (component-major, index
  * Y*z + y as the inflow profiles) accumulated, or NULL.  Geometry codes with
  * an adjoint: FLUID, PERIODIC, WALL, INFLOW, OUTFLOW_EQ, OUTFLOW_RIGHT,
- * NOTHING; other codes yield NaN at their sources.  Single block (no
- * overlaps).  The reference's adjoint machinery is for SRT
+ * NOTHING; other codes yield NaN at their sources, and so does an
+ * OUTFLOW_RIGHT site off the faces of the box.  Single block (no overlaps).
+ * The reference's adjoint machinery is for SRT
  * (d3q27/col_srt_adjoint.h:4-110, streaming_AB.h:246-281 streamingAdjoint,
  * bc.h:295-398, driven by sim_adjoint.cu:278-491); this entry point is the
  * same per-step operation for the default cumulant.                        */
 int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
                                const void* lambda_next, void* lambda, double* grad_inflow, void* stream);
+
+/* ---- moment recording for the adjoint -------------------------------------
+ * The Jacobian of the default cumulant step at a site depends on the pulled
+ * populations only through their ten raw moments (m0, j, Pi).  A forward
+ * step can record them, so that the adjoint reads 10 values per site instead
+ * of pulling 27 populations, and a run need not be recomputed backwards.
+ *
+ * tnl_lbm_d3q27_step_record: the same A-B step as tnl_lbm_d3q27_step over
+ * the whole local box (df[0] -> df[1], no macro output), with identical
+ * results, that also writes into `tape` (10 planes of XYZ elements: m0, jx,
+ * jy, jz, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz of the populations pulled at each
+ * FLUID, PERIODIC, OUTFLOW_RIGHT and OUTFLOW_EQ site; other sites are left
+ * untouched).  tape_f64 = 1: double elements (F64 storage only; the adjoint
+ * is then the same as from the state); 0: float elements.
+ * tnl_lbm_d3q27_adjoint_step_recorded: tnl_lbm_d3q27_adjoint_step with the
+ * Jacobian taken from a tape that tnl_lbm_d3q27_step_record wrote for that
+ * step; SD->df is not read.  No counterpart in the reference: its adjoint
+ * re-reads the stored primal macro fields (sim_adjoint.cu:189-241).        */
+int tnl_lbm_d3q27_step_record(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, void* tape, int tape_f64,
+                              void* stream);
+int tnl_lbm_d3q27_adjoint_step_recorded(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* tape,
+                                        int tape_f64, const void* lambda_next, void* lambda, double* grad_inflow,
+                                        void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,9 +1,13 @@
-"""The checkpointed run's bookkeeping (pytnl_lbm.adjoint.CheckpointedRun) on
-the CPU: the lattice step is replaced by an affine map f <- A f + B p whose
-adjoint is lam <- A^T lam, grad += B^T lam, so the gradient of any loss of
-the final state is fixed by torch autograd through the unrolled loop.  Every
-checkpoint stride K (1, sqrt T, T, non-dividing) must give that gradient, and
-the forward must return exactly the unrolled final state."""
+"""The bookkeeping of pytnl_lbm.adjoint.CheckpointedRun, on the CPU, for its
+three schedules.  The lattice step is replaced by the map
+    f <- A f + B p + c * pad(f[:w]^2)
+whose Jacobian at f is A + 2 c diag(pad(f[:w])): it depends on the state only
+through the first w = 10 n / 27 entries, which is what a recording holds
+(the ten raw moments per site of the real step).  The fake adjoint takes
+that part from the state ("recompute") or from the recording ("segment",
+"tape"), so a schedule that hands the adjoint the wrong step's state or
+recording gives a wrong gradient.  The gradient of a loss of the final state
+must equal torch autograd through the unrolled loop, for every stride K."""
 import types
 
 import numpy as np
@@ -13,38 +17,57 @@ import torch
 from pytnl_lbm import _abi
 from pytnl_lbm.adjoint import CheckpointedRun, lattice_run
 
-Z, Y, N = 3, 4, 40  # profile 3 x Z x Y = 36 values, state of 40 values
+Z, Y, N = 3, 4, 54  # profile 3 x Z x Y = 36 values, state of 54 values
+W = 10 * N // 27    # the part of the state a recording holds
+CQ = 0.05
 
 
-class AffineRun(CheckpointedRun):
-    def __init__(self, T, K, A, B):
-        cfg = types.SimpleNamespace(pattern=_abi.PATTERN_AB, collision=_abi.COLL_CUM, storage=_abi.STORE_F64)
-        d = types.SimpleNamespace(inflow_profile=[None, None, None])
-        lat = types.SimpleNamespace(cfg=cfg, d=d, df=[torch.zeros(N, dtype=torch.float64)], device=torch.device("cpu"),
-                                    X=1, Y=Y, Z=Z, _profiles=[None] * 3, _bind=lambda: None,
-                                    set_inflow=lambda *a, **k: None)
+def fake_lattice():
+    cfg = types.SimpleNamespace(pattern=_abi.PATTERN_AB, collision=_abi.COLL_CUM, storage=_abi.STORE_F64)
+    d = types.SimpleNamespace(inflow_profile=[None, None, None])
+    return types.SimpleNamespace(cfg=cfg, d=d, df=[torch.zeros(N, dtype=torch.float64)], device=torch.device("cpu"),
+                                 X=1, Y=Y, Z=Z, _profiles=[None] * 3, _bind=lambda: None,
+                                 set_inflow=lambda *a, **k: None)
+
+
+def quad(f):
+    out = torch.zeros_like(f)
+    out[..., :W] = CQ * f[..., :W] ** 2
+    return out
+
+
+class FakeRun(CheckpointedRun):
+    def __init__(self, T, K, A, B, mode):
         self.A, self.B, self.nsteps, self.nadj = A, B, 0, 0
-        super().__init__(lat, T, K)
+        super().__init__(fake_lattice(), T, K, mode=mode, tape_f64=True)
 
-    def _step(self, src, dst, stream=None):
+    def _step(self, src, dst, tape=None, stream=None):
         assert src.data_ptr() != dst.data_ptr()
-        dst.copy_(self.A @ src + self.B @ self.profile.reshape(-1))
+        if tape is not None:
+            tape.copy_(src[:W])
+        dst.copy_(self.A @ src + self.B @ self.profile.reshape(-1) + quad(src))
         self.nsteps += 1
 
-    def _adjoint(self, state, lam_next, lam, grad, stream=None):
+    def _adjoint(self, state, lam_next, lam, grad, tape=None, stream=None):
         assert lam_next.data_ptr() != lam.data_ptr()
-        lam.copy_(self.A.T @ lam_next)
+        part = state[:W] if tape is None else tape
+        assert (state is None) == (tape is not None)
+        out = self.A.T @ lam_next
+        out[:W] += 2 * CQ * part * lam_next[:W]
+        lam.copy_(out)
         grad += (self.B.T @ lam_next).reshape(grad.shape)
         self.nadj += 1
 
 
+@pytest.mark.parametrize("mode", ["recompute", "segment", "tape"])
 @pytest.mark.parametrize("T,K", [(1, None), (7, 1), (7, 7), (16, None), (17, None), (23, 5)])
-def test_checkpointed_gradient_equals_unrolled_autograd(T, K):
+def test_gradient_equals_unrolled_autograd(mode, T, K):
     g = torch.Generator().manual_seed(T * 31 + (K or 0))
-    A = torch.eye(N, dtype=torch.float64) + 0.1 * torch.randn(N, N, generator=g, dtype=torch.float64)
-    B = 0.3 * torch.randn(N, 3 * Z * Y, generator=g, dtype=torch.float64)
+    # a contraction, so that 23 steps of the quadratic map stay bounded
+    A = 0.7 * torch.eye(N, dtype=torch.float64) + 0.2 / N**0.5 * torch.randn(N, N, generator=g, dtype=torch.float64)
+    B = 0.05 * torch.randn(N, 3 * Z * Y, generator=g, dtype=torch.float64)
     p = torch.randn(3, Z, Y, generator=g, dtype=torch.float64)
-    f0 = torch.randn(N, generator=g, dtype=torch.float64)
+    f0 = 0.3 * torch.randn(N, generator=g, dtype=torch.float64)
     w = torch.randn(N, generator=g, dtype=torch.float64)
 
     def loss(fT):
@@ -53,26 +76,30 @@ def test_checkpointed_gradient_equals_unrolled_autograd(T, K):
     pr, fr = p.clone().requires_grad_(), f0.clone().requires_grad_()
     f = fr
     for _ in range(T):
-        f = A @ f + B @ pr.reshape(-1)
+        f = A @ f + B @ pr.reshape(-1) + quad(f)
     loss(f).backward()
 
-    run = AffineRun(T, K, A, B)
+    run = FakeRun(T, K, A, B, mode)
     pc, fc = p.clone().requires_grad_(), f0.clone().requires_grad_()
     fT = lattice_run(run, pc, fc)
-    assert torch.equal(fT.detach(), f.detach())
-    nfwd = run.nsteps
-    assert nfwd == T
+    assert torch.allclose(fT.detach(), f.detach(), rtol=1e-14, atol=1e-14)
+    assert run.nsteps == T
     loss(fT).backward()
     assert run.nadj == T
-    # recomputation: every step except the first of each segment
-    assert run.nsteps - nfwd == T - run.nseg
-    assert len(run.ckpt) == run.nseg and len(run.seg) == run.K - 1
-    np.testing.assert_allclose(pc.grad.numpy(), pr.grad.numpy(), rtol=1e-12, atol=1e-12 * pr.grad.abs().max().item())
-    np.testing.assert_allclose(fc.grad.numpy(), fr.grad.numpy(), rtol=1e-12, atol=1e-12 * fr.grad.abs().max().item())
+    recomputed = run.nsteps - T
+    if mode == "tape":
+        assert recomputed == 0 and run.tape.shape[0] == T and not run.ckpt
+    elif mode == "recompute":
+        assert recomputed == T - run.nseg and len(run.seg) == run.K - 1
+    else:  # every step of every segment is recorded once more
+        assert recomputed == T and run.tape.shape[0] == run.K
+    np.testing.assert_allclose(pc.grad.numpy(), pr.grad.numpy(), rtol=1e-11, atol=1e-11 * pr.grad.abs().max().item())
+    np.testing.assert_allclose(fc.grad.numpy(), fr.grad.numpy(), rtol=1e-11, atol=1e-11 * fr.grad.abs().max().item())
 
 
-def test_rejects_a_to_a_pattern():
+def test_rejects_a_to_a_pattern_and_unknown_modes():
     cfg = types.SimpleNamespace(pattern=_abi.PATTERN_AA, collision=_abi.COLL_CUM, storage=_abi.STORE_F64)
-    lat = types.SimpleNamespace(cfg=cfg)
     with pytest.raises(ValueError):
-        CheckpointedRun(lat, 10)
+        CheckpointedRun(types.SimpleNamespace(cfg=cfg), 10)
+    with pytest.raises(ValueError):
+        CheckpointedRun(fake_lattice(), 10, mode="revolve")

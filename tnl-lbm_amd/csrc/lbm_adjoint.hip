@@ -7,28 +7,53 @@ namespace lbm {
 
 namespace {
 
-template <int ST>
+template <int ST, int TAPE>
 int adjoint_for(const StepArgs& A0, const AdjPtrs& P, unsigned threads, unsigned grid, hipStream_t st)
 {
 	StepArgs A = A0;
 	linearize(A, threads, grid);
-	hipLaunchKernelGGL((k_adjoint_ab<ST, false>), dim3(grid), dim3(threads), 0, st, A, P);
+	hipLaunchKernelGGL((k_adjoint_ab<ST, false, TAPE>), dim3(grid), dim3(threads), 0, st, A, P);
 	if (int rc = hip_check(hipGetLastError(), "adjoint launch"))
 		return rc;
 	const uint64_t shell = shell_count(A.X, A.Y, A.Z);
 	const unsigned sgrid = (unsigned) ((shell + 255) / 256);
-	hipLaunchKernelGGL((k_adjoint_ab<ST, true>), dim3(sgrid), dim3(256), 0, st, A, P);
+	hipLaunchKernelGGL((k_adjoint_ab<ST, true, TAPE>), dim3(sgrid), dim3(256), 0, st, A, P);
 	return hip_check(hipGetLastError(), "adjoint fix-up launch");
+}
+
+template <int ST, int TAPE>
+int record_for(const StepArgs& A0, unsigned threads, unsigned grid, hipStream_t st)
+{
+	StepArgs A = A0;
+	linearize(A, threads, grid);
+	hipLaunchKernelGGL((k_general<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, TAPE>), dim3(grid), dim3(threads), 0, st, A);
+	return hip_check(hipGetLastError(), "recording step launch");
 }
 
 }  // namespace
 
-int launch_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdjPtrs& P, unsigned threads, unsigned grid,
-				   hipStream_t st)
+int launch_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdjPtrs& P, int tape_kind, unsigned threads,
+				   unsigned grid, hipStream_t st)
+{
+	if (cfg->storage == TNL_LBM_STORE_F64) {
+		if (tape_kind == 1)
+			return adjoint_for<TNL_LBM_STORE_F64, 1>(A, P, threads, grid, st);
+		if (tape_kind == 2)
+			return adjoint_for<TNL_LBM_STORE_F64, 2>(A, P, threads, grid, st);
+		return adjoint_for<TNL_LBM_STORE_F64, 0>(A, P, threads, grid, st);
+	}
+	if (tape_kind == 1)
+		return adjoint_for<TNL_LBM_STORE_F32, 1>(A, P, threads, grid, st);
+	return adjoint_for<TNL_LBM_STORE_F32, 0>(A, P, threads, grid, st);
+}
+
+int launch_record(const tnl_lbm_config* cfg, const StepArgs& A, int tape_kind, unsigned threads, unsigned grid,
+				  hipStream_t st)
 {
 	if (cfg->storage == TNL_LBM_STORE_F64)
-		return adjoint_for<TNL_LBM_STORE_F64>(A, P, threads, grid, st);
-	return adjoint_for<TNL_LBM_STORE_F32>(A, P, threads, grid, st);
+		return tape_kind == 2 ? record_for<TNL_LBM_STORE_F64, 2>(A, threads, grid, st)
+							  : record_for<TNL_LBM_STORE_F64, 1>(A, threads, grid, st);
+	return record_for<TNL_LBM_STORE_F32, 1>(A, threads, grid, st);
 }
 
 }  // namespace lbm

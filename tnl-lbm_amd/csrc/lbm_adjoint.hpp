@@ -22,6 +22,9 @@ namespace lbm {
 template <typename real>
 struct AdjCum
 {
+	static constexpr int NMOM = 10;
+	static LBM_DEV void raw_moments(const real* f, real* m) { raw_moments10(f, m); }
+
 	// forward backward-pass of one column, absolute form:
 	//   fz = k0 (1 - v^2) - 2 v k1 - k2,  f-+ = (k0 (v^2 -+ v) + k1 (2 v -+ 1) + k2) / 2
 	static LBM_DEV void bwd(real k0, real k1, real k2, real v, real& fm, real& fz, real& fp)
@@ -48,19 +51,18 @@ struct AdjCum
 	// divide by 1 while u and the trace keep the actual mass.
 	static LBM_DEV void vjp(const real* f, const real* lam, real Fx, real Fy, real Fz, real omega1, bool rho_fixed, real* mu)
 	{
+		real m[NMOM];
+		raw_moments(f, m);
+		vjp_m(m, lam, Fx, Fy, Fz, omega1, rho_fixed, mu);
+	}
+
+	// the same from the raw moments of the pre-collision populations, which
+	// are all the Jacobian of the default cumulant depends on
+	static LBM_DEV void vjp_m(const real* m, const real* lam, real Fx, real Fy, real Fz, real omega1, bool rho_fixed, real* mu)
+	{
 		// ---- forward recomputation ---------------------------------------
-		real m0 = 0, jx = 0, jy = 0, jz = 0, Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
-#pragma unroll
-		for (int q = 0; q < 27; q++) {
-			const real v = f[q];
-			m0 += v;
-			if (CX[q]) { jx += CX[q] > 0 ? v : -v; Pxx += v; }
-			if (CY[q]) { jy += CY[q] > 0 ? v : -v; Pyy += v; }
-			if (CZ[q]) { jz += CZ[q] > 0 ? v : -v; Pzz += v; }
-			if (CX[q] * CY[q]) Pxy += CX[q] * CY[q] > 0 ? v : -v;
-			if (CX[q] * CZ[q]) Pxz += CX[q] * CZ[q] > 0 ? v : -v;
-			if (CY[q] * CZ[q]) Pyz += CY[q] * CZ[q] > 0 ? v : -v;
-		}
+		const real m0 = m[0], jx = m[1], jy = m[2], jz = m[3], Pxx = m[4], Pyy = m[5], Pzz = m[6], Pxy = m[7], Pxz = m[8],
+				   Pyz = m[9];
 		const real hx = (real) 0.5 * Fx, hy = (real) 0.5 * Fy, hz = (real) 0.5 * Fz;
 		const real rv = (real) 1 / m0;
 		const real vx = (jx + hx) * rv, vy = (jy + hy) * rv, vz = (jz + hz) * rv;
@@ -232,6 +234,7 @@ struct AdjPtrs
 	const void* lam_next;  // adjoint of the state the forward step wrote (27 XYZ)
 	void* lam;             // adjoint of the state the forward step read (27 XYZ, output)
 	double* grad;          // d/d(inflow velocity), 3 x (Y Z) indexed Y z + y, accumulated; or null
+	const void* tape;      // raw moments recorded by the forward step (10 planes of XYZ), or null
 };
 
 // One thread per site.  Pass SPECIAL = false writes, for every site and q, the
@@ -290,8 +293,10 @@ LBM_DEV bool shell_coords(const StepArgs& A, int64_t& x, int64_t& z, int64_t& y)
 // periodic wrap (0 for slots whose true reader is elsewhere).  SPECIAL = true:
 // the face sites only, atomic adds of those contributions to their true
 // sources, after the first pass.
-template <int ST, bool SPECIAL>
-__global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjPtrs P)
+// (256, 2): two waves per SIMD; the FP64 form then spills a few registers
+// instead of running one wave per SIMD at 264
+template <int ST, bool SPECIAL, int TAPE = 0>
+__global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const AdjPtrs P)
 {
 	using S = Storage<ST>;
 	using real = typename S::real;
@@ -334,21 +339,39 @@ __global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjP
 			mu[q] = v;
 	}
 	else {
-		const elem* F = (const elem*) A.df0;
 		const elem* Ln = (const elem*) P.lam_next;
-		real f[27], lam[27];
+		real lam[27];
 #pragma unroll
-		for (int q = 0; q < 27; q++) {
+		for (int q = 0; q < 27; q++)
 			lam[q] = S::ld(Ln + q * A.XYZ + s0);
-			f[q] = S::ld(F + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+		// raw moments of the pulled populations: from the state, or recorded by the forward step
+		const bool needs_m = gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_EQ;
+		real m[D::NMOM];
+#pragma unroll
+		for (int k = 0; k < D::NMOM; k++)
+			m[k] = 0;
+		if (needs_m) {
+			if constexpr (TAPE == 0) {
+				const elem* F = (const elem*) A.df0;
+				real f[27];
+#pragma unroll
+				for (int q = 0; q < 27; q++)
+					f[q] = S::ld(F + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+				D::raw_moments(f, m);
+			}
+			else {
+				using TT = std::conditional_t<TAPE == 2, double, float>;
+				const TT* tp = (const TT*) P.tape;
+#pragma unroll
+				for (int k = 0; k < D::NMOM; k++)
+					m[k] = (real) tp[k * A.XYZ + s0];
+			}
 		}
 		switch (gi) {
 			case GEO_FLUID:
 			case GEO_PERIODIC:
-				D::vjp(f, lam, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, false, mu);
-				break;
 			case GEO_OUTFLOW_RIGHT:
-				D::vjp(f, lam, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, true, mu);
+				D::vjp_m(m, lam, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, gi == GEO_OUTFLOW_RIGHT, mu);
 				break;
 			case GEO_WALL:
 #pragma unroll
@@ -377,14 +400,7 @@ __global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjP
 			case GEO_OUTFLOW_EQ:
 				{
 					// f_out = feq(1, u(f_in)), u = (j + F/2) / m0
-					real m0 = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-					for (int q = 0; q < 27; q++) {
-						m0 += f[q];
-						jx += (real) CX[q] * f[q];
-						jy += (real) CY[q] * f[q];
-						jz += (real) CZ[q] * f[q];
-					}
+					const real m0 = m[0], jx = m[1], jy = m[2], jz = m[3];
 					const real rv = (real) 1 / m0;
 					const real ux = (jx + (real) 0.5 * (real) A.fx) * rv, uy = (jy + (real) 0.5 * (real) A.fy) * rv,
 							   uz = (jz + (real) 0.5 * (real) A.fz) * rv;
@@ -420,8 +436,12 @@ __global__ void __launch_bounds__(256) k_adjoint_ab(const StepArgs A, const AdjP
 	}
 }
 
-// both passes of one adjoint step (built in lbm_adjoint.hip)
-int launch_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdjPtrs& P, unsigned threads, unsigned grid,
-				   hipStream_t st);
+// both passes of one adjoint step (built in lbm_adjoint.hip); tape_kind 0:
+// Jacobian from the state df0, 1: from a float tape, 2: from a double tape
+int launch_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdjPtrs& P, int tape_kind, unsigned threads,
+				   unsigned grid, hipStream_t st);
+// one A-B step of the default cumulant that records the raw moments in A.tape
+int launch_record(const tnl_lbm_config* cfg, const StepArgs& A, int tape_kind, unsigned threads, unsigned grid,
+				  hipStream_t st);
 
 }  // namespace lbm

@@ -471,8 +471,8 @@ int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
 
 }  // extern "C"
 
-int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* lambda_next,
-							   void* lambda, double* grad_inflow, void* stream)
+// the configurations the adjoint and the moment recording are built for
+static int adjoint_config(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 {
 	if (! cfg || ! SD)
 		return fail(TNL_LBM_ERR_NULL, "null config or data");
@@ -482,13 +482,36 @@ int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the absolute storage forms (F64, F32)");
 	if (SD->ox || SD->oy || SD->oz)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for a single block without overlaps");
+	return TNL_LBM_OK;
+}
+
+static int tape_kind_of(const tnl_lbm_config* cfg, const void* tape, int tape_f64)
+{
+	if (! tape)
+		return fail(TNL_LBM_ERR_NULL, "moment tape missing");
+	if (tape_f64 && cfg->storage != TNL_LBM_STORE_F64)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "a double tape needs F64 storage");
+	return tape_f64 ? 2 : 1;
+}
+
+static int adjoint_impl(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* tape, int tape_kind,
+						const void* lambda_next, void* lambda, double* grad_inflow, void* stream)
+{
 	if (! lambda_next || ! lambda)
 		return fail(TNL_LBM_ERR_NULL, "adjoint arrays missing");
-	if (lambda == lambda_next || lambda == SD->df[0])
-		return fail(TNL_LBM_ERR_ARG, "lambda must not alias lambda_next or the state");
-	tnl_lbm_d3q27_data D = *SD;  // only df[0] is read; the pull step needs no second array
-	if (! D.df[1])
-		D.df[1] = D.df[0];
+	tnl_lbm_d3q27_data D = *SD;
+	if (tape_kind) {
+		// the state is not read: any non-null placeholder passes the layout checks
+		D.df[0] = D.df[1] = (void*) tape;
+		if (lambda == lambda_next || lambda == tape)
+			return fail(TNL_LBM_ERR_ARG, "lambda must not alias lambda_next or the tape");
+	}
+	else {
+		if (lambda == lambda_next || lambda == SD->df[0])
+			return fail(TNL_LBM_ERR_ARG, "lambda must not alias lambda_next or the state");
+		if (! D.df[1])
+			D.df[1] = D.df[0];  // only df[0] is read
+	}
 	StepArgs A;
 	unsigned threads = 0, grid = 0;
 	const int64_t off[3] = {0, 0, 0}, end[3] = {SD->X, SD->Y, SD->Z};
@@ -496,6 +519,47 @@ int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
 		return rc;
 	if (grid == 0)
 		return TNL_LBM_OK;
-	const AdjPtrs P{lambda_next, lambda, grad_inflow};
-	return launch_adjoint(cfg, A, P, threads, grid, (hipStream_t) stream);
+	const AdjPtrs P{lambda_next, lambda, grad_inflow, tape};
+	return launch_adjoint(cfg, A, P, tape_kind, threads, grid, (hipStream_t) stream);
+}
+
+int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* lambda_next,
+							   void* lambda, double* grad_inflow, void* stream)
+{
+	if (int rc = adjoint_config(cfg, SD))
+		return rc;
+	return adjoint_impl(cfg, SD, nullptr, 0, lambda_next, lambda, grad_inflow, stream);
+}
+
+int tnl_lbm_d3q27_step_record(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, void* tape, int tape_f64,
+							  void* stream)
+{
+	if (int rc = adjoint_config(cfg, SD))
+		return rc;
+	const int kind = tape_kind_of(cfg, tape, tape_f64);
+	if (kind < 0)
+		return kind;
+	if (tape == SD->df[0] || tape == SD->df[1])
+		return fail(TNL_LBM_ERR_ARG, "the tape must not alias the populations");
+	StepArgs A;
+	unsigned threads = 0, grid = 0;
+	const int64_t off[3] = {0, 0, 0}, end[3] = {SD->X, SD->Y, SD->Z};
+	if (int rc = make_step_args(cfg, SD, off, end, 0, A, threads, grid))
+		return rc;
+	if (grid == 0)
+		return TNL_LBM_OK;
+	A.tape = tape;
+	return launch_record(cfg, A, kind, threads, grid, (hipStream_t) stream);
+}
+
+int tnl_lbm_d3q27_adjoint_step_recorded(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* tape,
+										int tape_f64, const void* lambda_next, void* lambda, double* grad_inflow,
+										void* stream)
+{
+	if (int rc = adjoint_config(cfg, SD))
+		return rc;
+	const int kind = tape_kind_of(cfg, tape, tape_f64);
+	if (kind < 0)
+		return kind;
+	return adjoint_impl(cfg, SD, tape, kind, lambda_next, lambda, grad_inflow, stream);
 }

@@ -50,6 +50,7 @@ struct StepArgs
 	double fx, fy, fz;
 	double inflow[3];
 	const double* prof[3];
+	void* tape;  // moment recording of the A-B step (10 planes of XYZ), or null
 };
 
 constexpr int NXCD = 8;
@@ -368,7 +369,27 @@ LBM_DEV Nb neighbours(int64_t c, int64_t n, int64_t o, bool dist, bool periodic)
 	return Nb{m + o, c + o, p + o};
 }
 
-template <int ST, int COLL, int PATTERN>
+// The raw moments m0, j, Pi of 27 populations, in one summation order shared
+// by the step's moment recording and the adjoint that reads it back.
+template <typename real>
+LBM_DEV void raw_moments10(const real* f, real* m)
+{
+	real m0 = 0, jx = 0, jy = 0, jz = 0, Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
+#pragma unroll
+	for (int q = 0; q < 27; q++) {
+		const real v = f[q];
+		m0 += v;
+		if (CX[q]) { jx += CX[q] > 0 ? v : -v; Pxx += v; }
+		if (CY[q]) { jy += CY[q] > 0 ? v : -v; Pyy += v; }
+		if (CZ[q]) { jz += CZ[q] > 0 ? v : -v; Pzz += v; }
+		if (CX[q] * CY[q]) Pxy += CX[q] * CY[q] > 0 ? v : -v;
+		if (CX[q] * CZ[q]) Pxz += CX[q] * CZ[q] > 0 ? v : -v;
+		if (CY[q] * CZ[q]) Pyz += CY[q] * CZ[q] > 0 ? v : -v;
+	}
+	m[0] = m0; m[1] = jx; m[2] = jy; m[3] = jz; m[4] = Pxx; m[5] = Pyy; m[6] = Pzz; m[7] = Pxy; m[8] = Pxz; m[9] = Pyz;
+}
+
+template <int ST, int COLL, int PATTERN, int TAPE = 0>
 struct General
 {
 	using S = Storage<ST>;
@@ -466,6 +487,18 @@ struct General
 					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs));
 			}
 
+			if constexpr (TAPE != 0) {
+				// record the raw moments of the pulled populations (float or double tape)
+				if (gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_EQ) {
+					using TT = std::conditional_t<TAPE == 2, double, float>;
+					real m[10];
+					raw_moments10(f, m);
+					TT* __restrict__ tp = (TT*) A.tape;
+#pragma unroll
+					for (int k = 0; k < 10; k++)
+						tp[k * A.XYZ + s0] = (TT) m[k];
+				}
+			}
 			const real rest = S::dev ? (real) 1 : (real) 0;
 			switch (gi) {
 				case GEO_INFLOW:
@@ -620,13 +653,13 @@ struct General
 	}
 };
 
-template <int ST, int COLL, int PATTERN>
+template <int ST, int COLL, int PATTERN, int TAPE = 0>
 __global__ void __launch_bounds__(256) k_general(const StepArgs A)
 {
 	int64_t x, z, y;
 	if (! block_coords(A, x, z, y))
 		return;
-	General<ST, COLL, PATTERN>::site(A, x, y, z);
+	General<ST, COLL, PATTERN, TAPE>::site(A, x, y, z);
 }
 
 // ===========================================================================

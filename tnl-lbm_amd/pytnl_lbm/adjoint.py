@@ -48,67 +48,117 @@ def density_velocity(f: torch.Tensor, force=(0.0, 0.0, 0.0)):
     return rho, (j + 0.5 * F) / rho
 
 
+MODES = ("recompute", "segment", "tape")
+
+
 class CheckpointedRun:
-    """T A-B steps of a lattice from a given initial state, with a checkpoint
-    every K steps (default K = ceil(sqrt(T))).
+    """T A-B steps of a lattice from a given initial state, differentiable
+    through the hand-written adjoint.
 
-    The inflow velocity used by every step is the lattice's inflow profile
-    (NSE_Data_InflowProfile), which ``forward`` sets from a (3, Z, Y) float64
-    control tensor.  Storage F64 or F32; collision CUM."""
+    The Jacobian of a step at a site depends on the pulled populations only
+    through their ten raw moments, so the backward pass needs either the
+    states or a recording of those moments.  Three schedules:
 
-    def __init__(self, lat: D3Q27Lattice, T: int, K: int | None = None):
+    - ``"segment"`` (default): a checkpoint every K steps (K = ceil(sqrt(T)));
+      backwards, each segment is recomputed from its checkpoint by steps that
+      record the moments (tnl_lbm_d3q27_step_record), and the adjoint steps
+      read that recording.  Memory: T/K states + K recordings of 10 values.
+    - ``"recompute"``: the same checkpoints; the segment's states themselves
+      are kept and the adjoint pulls from them.  Memory: T/K + K states.
+    - ``"tape"``: every forward step records its moments; the backward pass
+      reads them and recomputes nothing.  Memory: T recordings (float32 by
+      default: 40 bytes per site and step).
+
+    Recordings are float64 for F64 storage in ``"segment"`` mode (the
+    gradient then equals the ``"recompute"`` one) and float32 otherwise,
+    unless ``tape_f64`` says.  The inflow velocity of every step is the
+    lattice's inflow profile (NSE_Data_InflowProfile), set by ``forward``
+    from a (3, Z, Y) float64 control tensor.  Storage F64 or F32; collision
+    CUM."""
+
+    def __init__(self, lat: D3Q27Lattice, T: int, K: int | None = None, mode: str = "segment",
+                 tape_f64: bool | None = None):
         if lat.cfg.pattern != _abi.PATTERN_AB:
             raise ValueError("the adjoint differentiates the A-B step: build the lattice with pattern='AB'")
         if lat.cfg.collision != _abi.COLL_CUM or lat.cfg.storage not in (_abi.STORE_F64, _abi.STORE_F32):
             raise ValueError("the adjoint supports the default cumulant with F64 or F32 storage")
         if T < 1:
             raise ValueError("T must be at least 1")
-        self.lat, self.T = lat, T
+        if mode not in MODES:
+            raise ValueError(f"mode must be one of {MODES}")
+        f64 = lat.cfg.storage == _abi.STORE_F64
+        if tape_f64 is None:
+            tape_f64 = f64 and mode == "segment"
+        if tape_f64 and not f64:
+            raise ValueError("a float64 recording needs F64 storage")
+        self.lat, self.T, self.mode, self.tape_f64 = lat, T, mode, bool(tape_f64)
         self.K = K or max(1, math.ceil(math.sqrt(T)))
-        self.nseg = math.ceil(T / self.K)
+        self.nseg = math.ceil(T / self.K) if mode != "tape" else 1
         el = lat.df[0].dtype
         n = lat.df[0].numel()
         dev = lat.device
+        tdt = torch.float64 if self.tape_f64 else torch.float32
         # every buffer starts zeroed so that slots a step does not write hold the same values everywhere
-        self.ckpt = [torch.zeros(n, dtype=el, device=dev) for _ in range(self.nseg)]
-        self.pp = [torch.zeros(n, dtype=el, device=dev) for _ in range(2)]
-        self.seg = [torch.zeros(n, dtype=el, device=dev) for _ in range(self.K - 1)]
-        self.lam = [torch.zeros(n, dtype=el, device=dev) for _ in range(2)]
+        z = lambda: torch.zeros(n, dtype=el, device=dev)  # noqa: E731
+        self.ckpt = [z() for _ in range(self.nseg)] if mode != "tape" else []
+        self.pp = [z() for _ in range(2)]
+        self.seg = [z() for _ in range(self.K - 1)] if mode == "recompute" else []
+        ntape = {"recompute": 0, "segment": self.K, "tape": T}[mode]
+        self.tape = torch.zeros(ntape, 10 * n // 27, dtype=tdt, device=dev) if ntape else None
+        self.lam = [z() for _ in range(2)]
         self.profile = torch.zeros(3, lat.Z, lat.Y, dtype=torch.float64, device=dev)
         lat.set_inflow(0.0, 0.0, 0.0, profile=[self.profile[0], self.profile[1], self.profile[2]])
-        # set_inflow copies non-contiguous input; bind the control buffer itself
-        for k in range(3):
+        for k in range(3):  # bind the control buffer itself
             lat._profiles[k] = self.profile[k]
             lat.d.inflow_profile[k] = self.profile[k].data_ptr()
 
-    def _step(self, src: torch.Tensor, dst: torch.Tensor, stream=None):
+    # ---- single launches ------------------------------------------------------
+    def _step(self, src: torch.Tensor, dst: torch.Tensor, tape: torch.Tensor | None = None, stream=None):
         lat = self.lat
         lat.d.even_iter = 1
         lat.d.df[0], lat.d.df[1] = src.data_ptr(), dst.data_ptr()
-        lat._check(lat.lib.tnl_lbm_d3q27_step(C.byref(lat.cfg), C.byref(lat.d), _abi.I64x3(0, 0, 0),
-                                              _abi.I64x3(lat.X, lat.Y, lat.Z), 0, _stream_ptr(stream)))
+        if tape is None:
+            lat._check(lat.lib.tnl_lbm_d3q27_step(C.byref(lat.cfg), C.byref(lat.d), _abi.I64x3(0, 0, 0),
+                                                  _abi.I64x3(lat.X, lat.Y, lat.Z), 0, _stream_ptr(stream)))
+        else:
+            lat._check(lat.lib.tnl_lbm_d3q27_step_record(C.byref(lat.cfg), C.byref(lat.d), _ptr(tape),
+                                                         int(self.tape_f64), _stream_ptr(stream)))
 
-    def _adjoint(self, state: torch.Tensor, lam_next: torch.Tensor, lam: torch.Tensor, grad: torch.Tensor,
-                 stream=None):
+    def _adjoint(self, state: torch.Tensor | None, lam_next: torch.Tensor, lam: torch.Tensor, grad: torch.Tensor,
+                 tape: torch.Tensor | None = None, stream=None):
         lat = self.lat
-        lat.d.df[0], lat.d.df[1] = state.data_ptr(), None
-        lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step(C.byref(lat.cfg), C.byref(lat.d), _ptr(lam_next), _ptr(lam),
-                                                      _ptr(grad), _stream_ptr(stream)))
+        if tape is None:
+            lat.d.df[0], lat.d.df[1] = state.data_ptr(), None
+            lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step(C.byref(lat.cfg), C.byref(lat.d), _ptr(lam_next),
+                                                          _ptr(lam), _ptr(grad), _stream_ptr(stream)))
+        else:
+            lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step_recorded(C.byref(lat.cfg), C.byref(lat.d), _ptr(tape),
+                                                                   int(self.tape_f64), _ptr(lam_next), _ptr(lam),
+                                                                   _ptr(grad), _stream_ptr(stream)))
 
+    def _other(self, src):
+        return self.pp[1] if src is self.pp[0] else self.pp[0]
+
+    # ---- the run ----------------------------------------------------------------
     def forward(self, inflow: torch.Tensor, f0: torch.Tensor) -> torch.Tensor:
         """Runs T steps from f0 (flat storage layout) with the given inflow
         profile; returns the buffer holding the final state (valid until the
         next call)."""
         try:
             self.profile.copy_(inflow.reshape(3, self.lat.Z, self.lat.Y))
+            if self.mode == "tape":
+                src = self.pp[0]
+                src.copy_(f0.reshape(-1))
+                for t in range(self.T):
+                    dst = self._other(src)
+                    self._step(src, dst, self.tape[t])
+                    src = dst
+                return src
             self.ckpt[0].copy_(f0.reshape(-1))
             src = self.ckpt[0]
             for t in range(self.T):
                 nxt = t + 1
-                if nxt % self.K == 0 and nxt < self.T:
-                    dst = self.ckpt[nxt // self.K]
-                else:
-                    dst = self.pp[1] if src is self.pp[0] else self.pp[0]
+                dst = self.ckpt[nxt // self.K] if nxt % self.K == 0 and nxt < self.T else self._other(src)
                 self._step(src, dst)
                 src = dst
             return src
@@ -117,19 +167,35 @@ class CheckpointedRun:
 
     def backward(self, lam_T: torch.Tensor, grad: torch.Tensor) -> torch.Tensor:
         """lam_T = dL/d(final state); adds dL/d(inflow) to grad (3, Z, Y) float64
-        and returns dL/d(initial state) (a buffer valid until the next call)."""
+        and returns dL/d(initial state) (a buffer valid until the next call).
+        Needs the recording of the last ``forward`` in ``"tape"`` mode."""
         try:
             cur, other = self.lam
             cur.copy_(lam_T.reshape(-1))
+            if self.mode == "tape":
+                for t in reversed(range(self.T)):
+                    self._adjoint(None, cur, other, grad, self.tape[t])
+                    cur, other = other, cur
+                return cur
             for j in reversed(range(self.nseg)):
                 t0 = j * self.K
                 m = min(self.T, t0 + self.K) - t0
-                states = [self.ckpt[j]] + self.seg[:m - 1]
-                for i in range(m - 1):
-                    self._step(states[i], states[i + 1])
-                for i in reversed(range(m)):
-                    self._adjoint(states[i], cur, other, grad)
-                    cur, other = other, cur
+                if self.mode == "recompute":
+                    states = [self.ckpt[j]] + self.seg[:m - 1]
+                    for i in range(m - 1):
+                        self._step(states[i], states[i + 1])
+                    for i in reversed(range(m)):
+                        self._adjoint(states[i], cur, other, grad)
+                        cur, other = other, cur
+                else:
+                    src = self.ckpt[j]
+                    for i in range(m):
+                        dst = self._other(src)
+                        self._step(src, dst, self.tape[i])
+                        src = dst
+                    for i in reversed(range(m)):
+                        self._adjoint(None, cur, other, grad, self.tape[i])
+                        cur, other = other, cur
             return cur
         finally:
             self.lat._bind()
