@@ -309,10 +309,11 @@ int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
  *
  * tnl_lbm_d3q27_step_record: the same A-B step as tnl_lbm_d3q27_step over
  * the whole local box (df[0] -> df[1], no macro output), with identical
- * results, that also writes into `tape` (10 planes of XYZ elements: m0, jx,
- * jy, jz, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz of the populations pulled at each
- * FLUID, PERIODIC, OUTFLOW_RIGHT and OUTFLOW_EQ site; other sites are left
- * untouched).  tape_f64 = 1: double elements (F64 storage only; the adjoint
+ * results, that also writes into `tape` the moments m0, jx, jy, jz, Pxx,
+ * Pyy, Pzz, Pxy, Pxz, Pyz (k = 0..9) of the populations pulled at each
+ * FLUID, PERIODIC, OUTFLOW_RIGHT and OUTFLOW_EQ site s (other sites are left
+ * untouched), at element ((s / 64) * 10 + k) * 64 + s % 64: blocks of 64
+ * sites, ceil(XYZ / 64) * 640 elements in all.  tape_f64 = 1: double elements (F64 storage only; the adjoint
  * is then the same as from the state); 0: float elements.
  * tnl_lbm_d3q27_adjoint_step_recorded: tnl_lbm_d3q27_adjoint_step with the
  * Jacobian taken from a tape that tnl_lbm_d3q27_step_record wrote for that

@@ -17,14 +17,14 @@ import torch
 from pytnl_lbm import _abi
 from pytnl_lbm.adjoint import CheckpointedRun, lattice_run
 
-Z, Y, N = 3, 4, 54  # profile 3 x Z x Y = 36 values, state of 54 values
-W = 10 * N // 27    # the part of the state a recording holds
+Z, Y, N = 3, 4, 27 * 64  # profile 3 x Z x Y = 36 values, state of 27 x 64 values
+W = 640                  # the part of the state the fake records: one step's recording for 64 sites
 CQ = 0.05
 
 
 def fake_lattice():
     cfg = types.SimpleNamespace(pattern=_abi.PATTERN_AB, collision=_abi.COLL_CUM, storage=_abi.STORE_F64)
-    d = types.SimpleNamespace(inflow_profile=[None, None, None])
+    d = types.SimpleNamespace(inflow_profile=[None, None, None], XYZ=N // 27)
     return types.SimpleNamespace(cfg=cfg, d=d, df=[torch.zeros(N, dtype=torch.float64)], device=torch.device("cpu"),
                                  X=1, Y=Y, Z=Z, _profiles=[None] * 3, _bind=lambda: None,
                                  set_inflow=lambda *a, **k: None)

@@ -207,7 +207,8 @@ def test_recording_step_equals_the_step_and_adjoint_from_tape(case, storage, tap
     g._check(g.lib.tnl_lbm_d3q27_step(C.byref(g.cfg), C.byref(g.d), _abi.I64x3(0, 0, 0), _abi.I64x3(X, Y, Z), 0,
                                       _stream_ptr()))
     plain = g.df[1].clone()
-    tape = torch.zeros(10 * n, dtype=torch.float64 if tape_f64 else torch.float32, device="cuda")
+    from pytnl_lbm.adjoint import tape_elements, tape_moments
+    tape = torch.zeros(tape_elements(n), dtype=torch.float64 if tape_f64 else torch.float32, device="cuda")
     out = torch.zeros_like(plain)
     g.d.df[1] = out.data_ptr()
     g._check(g.lib.tnl_lbm_d3q27_step_record(C.byref(g.cfg), C.byref(g.d), _ptr(tape), tape_f64, _stream_ptr()))
@@ -222,7 +223,7 @@ def test_recording_step_equals_the_step_and_adjoint_from_tape(case, storage, tap
                *(np.tensordot(c[k] * c[k], pulled, 1) for k in range(3)),
                np.tensordot(c[0] * c[1], pulled, 1), np.tensordot(c[0] * c[2], pulled, 1),
                np.tensordot(c[1] * c[2], pulled, 1)]
-        got = tape.double().cpu().numpy().reshape(10, X, Z, Y).transpose(0, 1, 3, 2)
+        got = tape_moments(tape, n).double().cpu().numpy().reshape(10, X, Z, Y).transpose(0, 1, 3, 2)
         tol = 1e-14 if (storage == "f64" and tape_f64) else 2e-6  # float: 27 rounded additions
         for k in range(10):
             assert np.abs(got[k] - mom[k]).max() <= tol * max(1.0, np.abs(mom[k]).max()), k

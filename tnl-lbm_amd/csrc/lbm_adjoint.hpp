@@ -234,7 +234,7 @@ struct AdjPtrs
 	const void* lam_next;  // adjoint of the state the forward step wrote (27 XYZ)
 	void* lam;             // adjoint of the state the forward step read (27 XYZ, output)
 	double* grad;          // d/d(inflow velocity), 3 x (Y Z) indexed Y z + y, accumulated; or null
-	const void* tape;      // raw moments recorded by the forward step (10 planes of XYZ), or null
+	const void* tape;      // raw moments recorded by the forward step (tape_index layout), or null
 };
 
 // One thread per site.  Pass SPECIAL = false writes, for every site and q, the
@@ -327,6 +327,20 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 	}
 	if (SPECIAL && (gi == GEO_NOTHING || irregular == 0))
 		return;
+	if (SPECIAL && (gi == GEO_WALL || gi == GEO_INFLOW)) {
+		// bounce-back: mu_q = lam_next[opp(q)], read only for the irregular directions
+		// (the face sites are one per cache line here); inflow: mu = 0
+		if (gi == GEO_WALL) {
+			const elem* Ln = (const elem*) P.lam_next;
+			elem* L = (elem*) P.lam;
+#pragma unroll
+			for (int q = 0; q < 27; q++)
+				if ((irregular >> q) & 1u)
+					atomicAdd(L + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]),
+							  (elem) S::ld(Ln + opp(q) * A.XYZ + s0));
+		}
+		return;
+	}
 	const bool on_shell = x == 0 || x == A.X - 1 || y == 0 || y == A.Y - 1 || z == 0 || z == A.Z - 1;
 	// an irregular pull away from the faces (OUTFLOW_RIGHT inside the box) has no fix-up: poison it
 	const bool unsupported = gi != GEO_NOTHING && irregular != 0 && ! on_shell;
@@ -364,7 +378,7 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 				const TT* tp = (const TT*) P.tape;
 #pragma unroll
 				for (int k = 0; k < D::NMOM; k++)
-					m[k] = (real) tp[k * A.XYZ + s0];
+					m[k] = (real) tp[tape_index(k, s0)];
 			}
 		}
 		switch (gi) {

@@ -369,6 +369,14 @@ LBM_DEV Nb neighbours(int64_t c, int64_t n, int64_t o, bool dist, bool periodic)
 	return Nb{m + o, c + o, p + o};
 }
 
+// Moment recording layout: blocks of 64 consecutive sites, each holding its
+// 10 moments as 10 runs of 64 values, so that one wave writes (and the
+// adjoint reads) one contiguous 2560-element stretch instead of 10 planes.
+__host__ __device__ inline int64_t tape_index(int k, int64_t s)
+{
+	return ((s >> 6) * 10 + k) * 64 + (s & 63);
+}
+
 // The raw moments m0, j, Pi of 27 populations, in one summation order shared
 // by the step's moment recording and the adjoint that reads it back.
 template <typename real>
@@ -487,17 +495,16 @@ struct General
 					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs));
 			}
 
+			// the raw moments of the pulled populations, recorded after the step's own stores
+			using TT = std::conditional_t<TAPE == 2, double, float>;
+			TT tm[10];
+			const bool rec = TAPE != 0 && (gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_EQ);
 			if constexpr (TAPE != 0) {
-				// record the raw moments of the pulled populations (float or double tape)
-				if (gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_EQ) {
-					using TT = std::conditional_t<TAPE == 2, double, float>;
-					real m[10];
-					raw_moments10(f, m);
-					TT* __restrict__ tp = (TT*) A.tape;
+				real m[10];
+				raw_moments10(f, m);
 #pragma unroll
-					for (int k = 0; k < 10; k++)
-						tp[k * A.XYZ + s0] = (TT) m[k];
-				}
+				for (int k = 0; k < 10; k++)
+					tm[k] = (TT) m[k];
 			}
 			const real rest = S::dev ? (real) 1 : (real) 0;
 			switch (gi) {
@@ -643,6 +650,14 @@ struct General
 #pragma unroll
 				for (int q = 0; q < 27; q++)
 					S::st(out + q * A.XYZ + s0, f[q], fx_inv(q, A.invr));
+			}
+			if constexpr (TAPE != 0) {
+				if (rec) {
+					TT* __restrict__ tp = (TT*) A.tape;
+#pragma unroll
+					for (int k = 0; k < 10; k++)
+						tp[tape_index(k, s0)] = tm[k];
+				}
 			}
 		}
 		if (A.out_macro)

@@ -51,6 +51,18 @@ def density_velocity(f: torch.Tensor, force=(0.0, 0.0, 0.0)):
 MODES = ("recompute", "segment", "tape")
 
 
+def tape_elements(XYZ: int) -> int:
+    """Elements of one step's moment recording: blocks of 64 sites x 10 moments."""
+    return (XYZ + 63) // 64 * 640
+
+
+def tape_moments(tape: torch.Tensor, XYZ: int) -> torch.Tensor:
+    """One step's recording -> (10, XYZ) view order (m0, jx, jy, jz, Pxx, Pyy,
+    Pzz, Pxy, Pxz, Pyz)."""
+    nb = (XYZ + 63) // 64
+    return tape.view(nb, 10, 64).permute(1, 0, 2).reshape(10, nb * 64)[:, :XYZ]
+
+
 class CheckpointedRun:
     """T A-B steps of a lattice from a given initial state, differentiable
     through the hand-written adjoint.
@@ -104,7 +116,7 @@ class CheckpointedRun:
         self.pp = [z() for _ in range(2)]
         self.seg = [z() for _ in range(self.K - 1)] if mode == "recompute" else []
         ntape = {"recompute": 0, "segment": self.K, "tape": T}[mode]
-        self.tape = torch.zeros(ntape, 10 * n // 27, dtype=tdt, device=dev) if ntape else None
+        self.tape = torch.zeros(ntape, tape_elements(lat.d.XYZ), dtype=tdt, device=dev) if ntape else None
         self.lam = [z() for _ in range(2)]
         self.profile = torch.zeros(3, lat.Z, lat.Y, dtype=torch.float64, device=dev)
         lat.set_inflow(0.0, 0.0, 0.0, profile=[self.profile[0], self.profile[1], self.profile[2]])
