@@ -321,6 +321,28 @@ int tnl_lbm_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
  * re-reads the stored primal macro fields (sim_adjoint.cu:189-241).        */
 int tnl_lbm_d3q27_step_record(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, void* tape, int tape_f64,
                               void* stream);
+/* ---- adjoint of the D3Q7 part of one coupled step ---------------------------
+ * The passive scalar of tnl_lbm_nse_ade_step (A-B pattern) does not act on
+ * the flow, and for a given velocity its update is linear in the D3Q7
+ * populations, so its adjoint needs only the velocity the step used:
+ *   lambda      <- (d g_out / d g_in)^T lambda_next
+ *   grad_source += (d g_out / d S)^T lambda_next = sum_q w_q lambda_next_q at
+ *                  FLUID, PERIODIC, SOLID and OUTFLOW_RIGHT sites
+ * velocity: vx, vy, vz as 3 consecutive planes of XYZ elements of
+ * cfg->macro_real, as the step wrote them in planes 1..3 of the NSE macro
+ * array with TNL_LBM_FLAG_OUTPUT_MACRO.  lambda_next / lambda: 7*XYZ elements
+ * of the compute type (double for F64 storage, float otherwise), layout of
+ * the populations; lambda must not alias lambda_next.  grad_source: XYZ
+ * doubles, accumulated, or NULL.  nse supplies sizes and the D3Q27 map (whose
+ * PERIODIC code decides the pull's wrap, as in the coupled kernel); its
+ * populations are not read.  D3Q7 codes with an adjoint: FLUID, PERIODIC,
+ * SOLID, WALL, INFLOW, OUTFLOW_RIGHT (on the faces of the box), NOTHING;
+ * others yield NaN.  Single block.  No counterpart in the reference (its
+ * adjoint is for the D3Q27 SRT flow, sim_adjoint.cu); the forward step it
+ * reverses is kernels.h:114-196 with d3q7/col_*.h and d3q7/bc.h.           */
+int tnl_lbm_d3q7_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* nse, const tnl_lbm_d3q7_data* ade,
+                              const void* velocity, const void* lambda_next, void* lambda, double* grad_source,
+                              void* stream);
 int tnl_lbm_d3q27_adjoint_step_recorded(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const void* tape,
                                         int tape_f64, const void* lambda_next, void* lambda, double* grad_inflow,
                                         void* stream);

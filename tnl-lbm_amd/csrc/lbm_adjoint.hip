@@ -30,7 +30,29 @@ int record_for(const StepArgs& A0, unsigned threads, unsigned grid, hipStream_t 
 	return hip_check(hipGetLastError(), "recording step launch");
 }
 
+template <int ST>
+int ade_adjoint_for(const StepArgs& A0, const AdeAdjArgs& P, unsigned threads, unsigned grid, hipStream_t st)
+{
+	StepArgs A = A0;
+	linearize(A, threads, grid);
+	hipLaunchKernelGGL((k_ade_adjoint_ab<ST, false>), dim3(grid), dim3(threads), 0, st, A, P);
+	if (int rc = hip_check(hipGetLastError(), "D3Q7 adjoint launch"))
+		return rc;
+	const uint64_t shell = shell_count(A.X, A.Y, A.Z);
+	hipLaunchKernelGGL((k_ade_adjoint_ab<ST, true>), dim3((unsigned) ((shell + 255) / 256)), dim3(256), 0, st, A, P);
+	return hip_check(hipGetLastError(), "D3Q7 adjoint fix-up launch");
+}
+
 }  // namespace
+
+int launch_ade_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdeAdjArgs& P, unsigned threads,
+					   unsigned grid, hipStream_t st)
+{
+	// the compute type follows the storage: double for F64, float for the others
+	if (cfg->storage == TNL_LBM_STORE_F64)
+		return ade_adjoint_for<TNL_LBM_STORE_F64>(A, P, threads, grid, st);
+	return ade_adjoint_for<TNL_LBM_STORE_F32>(A, P, threads, grid, st);
+}
 
 int launch_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdjPtrs& P, int tape_kind, unsigned threads,
 				   unsigned grid, hipStream_t st)

@@ -450,6 +450,151 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 	}
 }
 
+// ===========================================================================
+// Adjoint of the D3Q7 part of one coupled A-B step (ade_site), for a given
+// velocity field: the passive scalar does not act on the flow, and the D3Q7
+// collision is linear in the populations for a given u (SRT, MRT and CLBM
+// relax moments whose equilibria are linear in phi), so
+//   lambda <- C(u)^T lambda_next scattered back along the pull,
+//   dJ/dS  += sum_q w_q lambda_next_q at the sites that collide.
+// The columns of C(u) are the forward collision applied to the unit vectors.
+// ===========================================================================
+struct AdeAdjArgs
+{
+	const void* u;          // vx, vy, vz planes (XYZ each) of the step's velocity, macro_real
+	const void* lam_next;   // 7 XYZ of the compute type
+	void* lam;              // 7 XYZ, output
+	double* grad_source;    // XYZ, accumulated, or null
+	const int16_t* map;     // D3Q7 codes, or null (all periodic)
+	const void* diff;       // per-site diffusivity (macro_real) or null
+	double D;
+	int32_t collision;
+};
+
+template <int ST, bool SPECIAL>
+__global__ void __launch_bounds__(256) k_ade_adjoint_ab(const StepArgs A, const AdeAdjArgs P)
+{
+	using K = Ade7<ST>;
+	using real = typename K::real;
+	int64_t x, z, y;
+	if (SPECIAL ? ! shell_coords(A, x, z, y) : ! block_coords(A, x, z, y))
+		return;
+	const int64_t s0 = (x * A.Zs + z) * A.Ys + y;
+	// the pull uses the NSE site's periodicity, as in the coupled kernel
+	const int16_t gn = A.map ? A.map[s0] : (int16_t) GEO_PERIODIC;
+	const bool per = gn == GEO_PERIODIC;
+	const int16_t ga = P.map ? P.map[s0] : (int16_t) A_PERIODIC;
+	const Nb NX = neighbours<false>(x, A.X, 0, false, per);
+	const Nb NY = neighbours<false>(y, A.Y, 0, false, per);
+	const Nb NZ = neighbours<false>(z, A.Z, 0, false, per);
+	Nb PX = NX;
+	if (ga == A_OUTFLOW_RIGHT)
+		PX.p = PX.o = NX.m;
+	const Nb WX = neighbours<false>(x, A.X, 0, false, true);
+	const Nb WY = neighbours<false>(y, A.Y, 0, false, true);
+	const Nb WZ = neighbours<false>(z, A.Z, 0, false, true);
+	auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
+	uint32_t irregular = 0;
+#pragma unroll
+	for (int q = 0; q < 7; q++) {
+		const bool same = PX[-CX[q]] == WX[-CX[q]] && NY[-CY[q]] == WY[-CY[q]] && NZ[-CZ[q]] == WZ[-CZ[q]];
+		if (ga == A_NOTHING || ! same)
+			irregular |= 1u << q;
+	}
+	if (SPECIAL && (ga == A_NOTHING || irregular == 0))
+		return;
+	const real* Ln = (const real*) P.lam_next;
+	real* L = (real*) P.lam;
+	if (SPECIAL && (ga == A_WALL || ga == A_INFLOW)) {
+#pragma unroll
+		for (int q = 0; q < 7; q++)
+			if (ga == A_WALL && ((irregular >> q) & 1u))
+				atomicAdd(L + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), Ln[opp(q) * A.XYZ + s0]);
+		return;
+	}
+	const bool on_shell = x == 0 || x == A.X - 1 || y == 0 || y == A.Y - 1 || z == 0 || z == A.Z - 1;
+	const bool unsupported = ga != A_NOTHING && irregular != 0 && ! on_shell;
+	real mu[7];
+	if (ga == A_NOTHING || unsupported) {
+		const real v = unsupported ? (real) __builtin_nan("") : (real) 0;
+#pragma unroll
+		for (int q = 0; q < 7; q++)
+			mu[q] = v;
+	}
+	else {
+		real lam[7];
+#pragma unroll
+		for (int q = 0; q < 7; q++)
+			lam[q] = Ln[q * A.XYZ + s0];
+		switch (ga) {
+			case A_FLUID:
+			case A_PERIODIC:
+			case A_SOLID:
+			case A_OUTFLOW_RIGHT:
+				{
+					auto macro_at = [&](const void* p, int64_t i) -> real {
+						return A.macro_f64 ? (real) ((const double*) p)[i] : (real) ((const float*) p)[i];
+					};
+					const real vx = macro_at(P.u, s0), vy = macro_at(P.u, A.XYZ + s0), vz = macro_at(P.u, 2 * A.XYZ + s0);
+					const real D = P.diff ? macro_at(P.diff, s0) : (real) P.D;
+#pragma unroll
+					for (int p = 0; p < 7; p++) {
+						real e[7];
+#pragma unroll
+						for (int q = 0; q < 7; q++)
+							e[q] = q == p ? (real) 1 : (real) 0;
+						K::collide(P.collision, e, (real) 1, vx, vy, vz, D);
+						real acc = 0;
+#pragma unroll
+						for (int q = 0; q < 7; q++)
+							acc += lam[q] * e[q];
+						mu[p] = acc;
+					}
+					if (! SPECIAL && P.grad_source) {
+						real g = 0;
+#pragma unroll
+						for (int q = 0; q < 7; q++)
+							g += (real) w7(q) * lam[q];
+						P.grad_source[s0] += (double) g;
+					}
+					break;
+				}
+			case A_WALL:
+#pragma unroll
+				for (int q = 0; q < 7; q++)
+					mu[q] = lam[opp(q)];
+				break;
+			case A_INFLOW:
+#pragma unroll
+				for (int q = 0; q < 7; q++)
+					mu[q] = 0;
+				break;
+			default:
+#pragma unroll
+				for (int q = 0; q < 7; q++)
+					mu[q] = (real) __builtin_nan("");
+				break;
+		}
+	}
+	if (! SPECIAL) {
+#pragma unroll
+		for (int q = 0; q < 7; q++) {
+			const real v = (irregular >> q) & 1u && ! unsupported ? (real) 0 : mu[q];
+			L[q * A.XYZ + sidx(WX[-CX[q]], WY[-CY[q]], WZ[-CZ[q]])] = v;
+		}
+	}
+	else {
+#pragma unroll
+		for (int q = 0; q < 7; q++)
+			if ((irregular >> q) & 1u)
+				atomicAdd(L + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), mu[q]);
+	}
+}
+
+// the D3Q7 adjoint step (built in lbm_adjoint.hip)
+int launch_ade_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdeAdjArgs& P, unsigned threads,
+					   unsigned grid, hipStream_t st);
+
 // both passes of one adjoint step (built in lbm_adjoint.hip); tape_kind 0:
 // Jacobian from the state df0, 1: from a float tape, 2: from a double tape
 int launch_adjoint(const tnl_lbm_config* cfg, const StepArgs& A, const AdjPtrs& P, int tape_kind, unsigned threads,

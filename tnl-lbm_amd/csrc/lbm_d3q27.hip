@@ -563,3 +563,37 @@ int tnl_lbm_d3q27_adjoint_step_recorded(const tnl_lbm_config* cfg, const tnl_lbm
 		return kind;
 	return adjoint_impl(cfg, SD, tape, kind, lambda_next, lambda, grad_inflow, stream);
 }
+
+int tnl_lbm_d3q7_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* nse, const tnl_lbm_d3q7_data* ade,
+							  const void* velocity, const void* lambda_next, void* lambda, double* grad_source,
+							  void* stream)
+{
+	if (! cfg || ! nse || ! ade)
+		return fail(TNL_LBM_ERR_NULL, "null config or data");
+	if (cfg->pattern != TNL_LBM_PATTERN_AB)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the D3Q7 adjoint is built for the A-B pattern");
+	if (nse->ox || nse->oy || nse->oz)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the D3Q7 adjoint is built for a single block without overlaps");
+	if (ade->struct_size != sizeof(tnl_lbm_d3q7_data) || ade->version != TNL_LBM_ABI_VERSION)
+		return fail(TNL_LBM_ERR_VERSION, "tnl_lbm_d3q7_data struct_size/version mismatch");
+	if (ade->collision < TNL_LBM_ADE_COLL_SRT || ade->collision > TNL_LBM_ADE_COLL_CLBM)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "unknown ADE collision");
+	if (ade->X != nse->X || ade->Y != nse->Y || ade->Z != nse->Z || ade->ox || ade->oy || ade->oz || ade->XYZ != nse->XYZ)
+		return fail(TNL_LBM_ERR_DIMS, "NSE and ADE lattices must have the same geometry");
+	if (! velocity || ! lambda_next || ! lambda)
+		return fail(TNL_LBM_ERR_NULL, "velocity or adjoint arrays missing");
+	if (lambda == lambda_next)
+		return fail(TNL_LBM_ERR_ARG, "lambda must not alias lambda_next");
+	tnl_lbm_d3q27_data D = *nse;
+	D.df[0] = D.df[1] = (void*) velocity;  // the populations are not read: placeholders for the layout checks
+	StepArgs A;
+	unsigned threads = 0, grid = 0;
+	const int64_t off[3] = {0, 0, 0}, end[3] = {nse->X, nse->Y, nse->Z};
+	if (int rc = make_step_args(cfg, &D, off, end, 0, A, threads, grid))
+		return rc;
+	if (grid == 0)
+		return TNL_LBM_OK;
+	const AdeAdjArgs P{velocity, lambda_next, lambda, grad_source, ade->map, ade->diffusion_coefficient,
+					   ade->lbm_diffusivity, ade->collision};
+	return launch_ade_adjoint(cfg, A, P, threads, grid, (hipStream_t) stream);
+}

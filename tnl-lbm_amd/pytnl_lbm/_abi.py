@@ -182,6 +182,7 @@ def _declare(lib, prefix: str):
         "d3q27_adjoint_step": [cfgp, d27p, _P, _P, _P],
         "d3q27_step_record": [cfgp, d27p, _P, C.c_int],
         "d3q27_adjoint_step_recorded": [cfgp, d27p, _P, C.c_int, _P, _P, _P],
+        "d3q7_adjoint_step": [cfgp, d27p, d7p, _P, _P, _P, _P],
     }
     for name, args in sig.items():
         fn = getattr(lib, prefix + name, None)
@@ -191,7 +192,7 @@ def _declare(lib, prefix: str):
         if prefix == "oracle_" and name in ("d3q27_set_equilibrium", "d3q27_set_equilibrium_field",
                                             "d3q27_compute_macro", "d3q27_export_f64", "d3q27_import_f64",
                                             "d3q7_set_equilibrium", "d3q7_export_f64", "d3q27_adjoint_step",
-                                            "d3q27_step_record", "d3q27_adjoint_step_recorded"):
+                                            "d3q27_step_record", "d3q27_adjoint_step_recorded", "d3q7_adjoint_step"):
             fn.argtypes = args  # the oracle's init/transfer calls take no thread count
         fn.restype = C.c_int
 

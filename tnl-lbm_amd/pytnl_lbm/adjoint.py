@@ -233,3 +233,163 @@ def lattice_run(run: CheckpointedRun, inflow: torch.Tensor, f0: torch.Tensor) ->
     populations (flat storage layout)) -> final populations (same layout).
     Gradients flow to both inputs through the hand-written adjoint."""
     return _LatticeRun.apply(inflow, f0, run)
+
+
+# ---- coupled NSE + passive scalar --------------------------------------------------
+
+class CoupledScalarRun:
+    """T coupled NSE + D3Q7 A-B steps (tnl_lbm_nse_ade_step, kernels.h:114-196)
+    from given initial states, differentiable with respect to the scalar's
+    source field.
+
+    The scalar does not act on the flow and, for a given velocity, its step is
+    linear in the D3Q7 populations, so the backward pass needs the velocity of
+    every step and nothing of the scalar's own trajectory.  The flow is
+    checkpointed every K steps (K = ceil(sqrt(T))); backwards, each segment is
+    recomputed by plain D3Q27 steps that write their velocity (macro output
+    into one buffer per step, the same velocity the coupled kernel hands to
+    the D3Q7 update), and the D3Q7 adjoint steps (tnl_lbm_d3q7_adjoint_step)
+    run over it in reverse.  Memory: T/K flow states + K velocity fields.
+    Any storage form; the adjoint runs in the compute type (double for F64,
+    float otherwise) and accumulates the source gradient in double."""
+
+    def __init__(self, nse: D3Q27Lattice, ade, T: int, K: int | None = None):
+        if nse.cfg.pattern != _abi.PATTERN_AB:
+            raise ValueError("the adjoint differentiates the A-B step: build the lattices with pattern='AB'")
+        if nse.cfg.collision != _abi.COLL_CUM:
+            raise ValueError("the coupled kernel is built for the default cumulant")
+        if ade.nse is not nse:
+            raise ValueError("the D3Q7 lattice belongs to another NSE lattice")
+        if T < 1:
+            raise ValueError("T must be at least 1")
+        self.nse, self.ade, self.T = nse, ade, T
+        self.K = K or max(1, math.ceil(math.sqrt(T)))
+        self.nseg = math.ceil(T / self.K)
+        dev, n = nse.device, nse.d.XYZ
+        el = nse.df[0].dtype
+        mdt = torch.float64 if nse.cfg.macro_real == _abi.REAL_F64 else torch.float32
+        cdt = torch.float64 if nse.cfg.storage == _abi.STORE_F64 else torch.float32
+        self.ckpt = [torch.zeros(27 * n, dtype=el, device=dev) for _ in range(self.nseg)]
+        self.pp = [torch.zeros(27 * n, dtype=el, device=dev) for _ in range(2)]
+        self.app = [torch.zeros(7 * n, dtype=el, device=dev) for _ in range(2)]
+        self.ubuf = [torch.zeros(4 * n, dtype=mdt, device=dev) for _ in range(self.K)]
+        self.lam = [torch.zeros(7 * n, dtype=cdt, device=dev) for _ in range(2)]
+        self.out = torch.zeros(7 * n, dtype=torch.float64, device=dev)
+        self.source = torch.zeros(n, dtype=mdt, device=dev)
+        ade.source = self.source
+        ade._bind()
+
+    def _launch(self, rc):
+        self.nse._check(rc)
+
+    def _coupled(self, fs, fd, gs, gd, stream=None):
+        nse, ade = self.nse, self.ade
+        nse.d.even_iter = ade.d.even_iter = 1
+        nse.d.df[0], nse.d.df[1] = fs.data_ptr(), fd.data_ptr()
+        ade.d.df[0], ade.d.df[1] = gs.data_ptr(), gd.data_ptr()
+        self._launch(nse.lib.tnl_lbm_nse_ade_step(C.byref(nse.cfg), C.byref(nse.d), C.byref(ade.d),
+                                                  _abi.I64x3(0, 0, 0), _abi.I64x3(nse.X, nse.Y, nse.Z), 0,
+                                                  _stream_ptr(stream)))
+
+    def _flow_step(self, fs, fd, ubuf, stream=None):
+        nse = self.nse
+        nse.d.even_iter = 1
+        nse.d.df[0], nse.d.df[1] = fs.data_ptr(), fd.data_ptr()
+        nse.d.macro = ubuf.data_ptr()
+        self._launch(nse.lib.tnl_lbm_d3q27_step(C.byref(nse.cfg), C.byref(nse.d), _abi.I64x3(0, 0, 0),
+                                                _abi.I64x3(nse.X, nse.Y, nse.Z), _abi.FLAG_OUTPUT_MACRO,
+                                                _stream_ptr(stream)))
+
+    def _adjoint(self, ubuf, lam_next, lam, grad, stream=None):
+        nse = self.nse
+        self._launch(nse.lib.tnl_lbm_d3q7_adjoint_step(C.byref(nse.cfg), C.byref(nse.d), C.byref(self.ade.d),
+                                                       _ptr(ubuf[nse.d.XYZ:]), _ptr(lam_next), _ptr(lam),
+                                                       _ptr(grad), _stream_ptr(stream)))
+
+    def export(self, g: torch.Tensor) -> torch.Tensor:
+        """A D3Q7 state (storage form) -> absolute populations, flat float64
+        (a buffer valid until the next call)."""
+        ade = self.ade
+        saved = ade.d.df[0]
+        ade.d.df[0] = g.data_ptr()
+        try:
+            ade._check(ade.lib.tnl_lbm_d3q7_export_f64(C.byref(ade.cfg), C.byref(ade.d), 0, _ptr(self.out),
+                                                       _stream_ptr()))
+        finally:
+            ade.d.df[0] = saved
+        return self.out
+
+    def forward(self, source: torch.Tensor, g0: torch.Tensor, f0: torch.Tensor) -> torch.Tensor:
+        """T coupled steps from (f0, g0) (flat storage layout) with the given
+        source (flat XYZ in the lattice layout); returns the buffer holding
+        the final D3Q7 state."""
+        try:
+            self.source.copy_(source.reshape(-1))
+            self.ckpt[0].copy_(f0.reshape(-1))
+            self.app[0].copy_(g0.reshape(-1))
+            fs, gs = self.ckpt[0], self.app[0]
+            for t in range(self.T):
+                nxt = t + 1
+                fd = self.ckpt[nxt // self.K] if nxt % self.K == 0 and nxt < self.T else (
+                    self.pp[1] if fs is self.pp[0] else self.pp[0])
+                gd = self.app[1] if gs is self.app[0] else self.app[0]
+                self._coupled(fs, fd, gs, gd)
+                fs, gs = fd, gd
+            return gs
+        finally:
+            self.nse._bind()
+            self.ade._bind()
+
+    def backward(self, lam_T: torch.Tensor, grad: torch.Tensor) -> torch.Tensor:
+        """lam_T = dL/d(final absolute D3Q7 populations); adds dL/d(source) to
+        grad (flat XYZ float64) and returns dL/d(initial absolute D3Q7
+        populations) (a buffer valid until the next call)."""
+        try:
+            cur, other = self.lam
+            cur.copy_(lam_T.reshape(-1))
+            for j in reversed(range(self.nseg)):
+                t0 = j * self.K
+                m = min(self.T, t0 + self.K) - t0
+                fs = self.ckpt[j]
+                for i in range(m):
+                    fd = self.pp[1] if fs is self.pp[0] else self.pp[0]
+                    self._flow_step(fs, fd, self.ubuf[i])
+                    fs = fd
+                for i in reversed(range(m)):
+                    self._adjoint(self.ubuf[i], cur, other, grad)
+                    cur, other = other, cur
+            return cur
+        finally:
+            self.nse._bind()
+            self.ade._bind()
+
+
+class _ScalarRun(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, source, run: CoupledScalarRun, g0, f0):
+        gT = run.forward(source.detach(), g0, f0)
+        ctx.run = run
+        ctx.shape = source.shape
+        return run.export(gT).clone()
+
+    @staticmethod
+    def backward(ctx, gG):
+        run = ctx.run
+        grad = torch.zeros(run.nse.d.XYZ, dtype=torch.float64, device=gG.device)
+        run.backward(gG.to(run.lam[0].dtype).contiguous(), grad)
+        return grad.reshape(ctx.shape), None, None, None
+
+
+def scalar_run(run: CoupledScalarRun, source: torch.Tensor, g0: torch.Tensor, f0: torch.Tensor) -> torch.Tensor:
+    """Differentiable coupled run: the source field (flat XYZ float64 in the
+    lattice layout) -> the final absolute D3Q7 populations (flat 7*XYZ
+    float64).  g0 and f0 are the initial D3Q7 and D3Q27 states in the storage
+    form; the gradient flows to the source."""
+    return _ScalarRun.apply(source, run, g0, f0)
+
+
+def top_layer_variance(g: torch.Tensor, lat: D3Q27Lattice, layers: int) -> torch.Tensor:
+    """Variance of phi = sum_q g_q over the top `layers` z-layers, in float64
+    (g: flat 7*XYZ absolute populations)."""
+    phi = g.view(7, lat.d.XYZ)[:, :lat.nsites].sum(0).view(lat.X, lat.Z, lat.Y)[:, lat.Z - layers:, :]
+    return ((phi - phi.mean()) ** 2).mean()
