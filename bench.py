@@ -42,6 +42,7 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--adjoint-steps", type=int, default=2000)
     p.add_argument("--adjoint-fd-dirs", type=int, default=16)
+    p.add_argument("--coupled-steps", type=int, default=1000)
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--storage", default="f64", choices=["f64", "f32", "f32dev", "f16dev", "bf16dev", "i16dev"],
                    help="storage form of the headline case (f64 is the headline workload; the others exist "
@@ -242,6 +243,79 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
     return res
 
 
+def coupled_adjoint_case(N, T, storage, layers=32, fd_dirs=0, grad_out=False):
+    """Coupled NSE + D3Q7 passive scalar on N^3, periodic in x and y, walls
+    in z; seeded turbulence (u_rms = 0.03, k0 = 6; zero on the walls),
+    tau = 0.505; D3Q7 CLBM with tau_ADE = 0.55 (D = 0.0125); source: 16
+    Gaussian blobs, sigma = 6, amplitude 1e-3.  Objective: FP64 variance of
+    the scalar over the top `layers` z-layers after T steps; gradient with
+    respect to the source through the checkpointed run."""
+    import numpy as np
+    import torch
+    from pytnl_lbm import (ADE_GEO, GEO, CoupledScalarRun, D3Q7Lattice, D3Q27Lattice, _abi, gaussian_blobs,
+                           scalar_run, to_storage, top_layer_variance)
+    from pytnl_lbm.fields import turbulence_fields_torch
+
+    geo = np.full((N, N, N), GEO.PERIODIC, dtype=np.int16)
+    geo[:, :, 0] = geo[:, :, -1] = GEO.WALL
+    ageo = np.full((N, N, N), ADE_GEO.PERIODIC, dtype=np.int16)
+    ageo[:, :, 0] = ageo[:, :, -1] = ADE_GEO.WALL
+    macro = _abi.REAL_F64 if storage == "f64" else _abi.REAL_F32
+    lat = D3Q27Lattice(N, N, N, storage=storage, pattern="AB", nu=(0.505 - 0.5) / 3, geo=geo, macro_real=macro)
+    ade = D3Q7Lattice(lat, collision="CLBM", diffusivity=(0.55 - 0.5) / 4, geo=ageo, phi_ref=0.0, store_scale=1000.0)
+    u = turbulence_fields_torch(N, 7, 6.0, 0.03, device="cuda")
+    for c in u:
+        c[:, :, 0] = 0
+        c[:, :, -1] = 0
+    lat.set_equilibrium_field(torch.ones_like(u[0]), *u)
+    del u
+    ade.set_equilibrium(0.0)
+    f0, g0 = lat.df[0].clone(), ade.df[0].clone()
+    src = to_storage(torch.tensor(gaussian_blobs(N, N, N, 16, 6.0, 1e-3, seed=11))).reshape(-1).to("cuda")
+    torch.cuda.empty_cache()
+    run = CoupledScalarRun(lat, ade, T)
+    sites = N ** 3
+
+    def J(s):
+        return top_layer_variance(scalar_run(run, s, g0, f0), lat, layers)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run.forward(src, g0, f0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    sr = src.clone().requires_grad_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Jv = J(sr)
+    Jv.backward()
+    torch.cuda.synchronize()
+    t_fb = time.perf_counter() - t0
+    res = {
+        "workload": f"coupled NSE + D3Q7 CLBM {N}^3, {storage.upper()}, periodic x,y, walls z, T={T}, "
+                    f"J = variance of phi over the top {layers} layers",
+        "forward_mlups": round(sites * T / t_fwd / 1e6, 1),
+        "fwd_bwd_mlups": round(sites * T / t_fb / 1e6, 1),
+        "ratio": round(t_fwd / t_fb, 4),
+        "checkpoint_stride": run.K,
+        "J": Jv.item(),
+    }
+    grad = sr.grad.clone()
+    if fd_dirs:
+        gen = torch.Generator(device="cuda").manual_seed(13)
+        worst = 0.0
+        with torch.no_grad():
+            for _ in range(fd_dirs):
+                d = torch.randn(src.shape, generator=gen, device="cuda", dtype=torch.float64) * 1e-3
+                eps = 1e-2
+                fd = (J(src + eps * d) - J(src - eps * d)).item() / (2 * eps)
+                worst = max(worst, abs((grad * d).sum().item() - fd) / abs(fd))
+        res["fd_check"] = {"eps": 1e-2, "directions": fd_dirs, "max_rel_err": float(f"{worst:.3e}")}
+    del run, ade, lat
+    torch.cuda.empty_cache()
+    return (res, grad) if grad_out else res
+
+
 def cpu_baseline(seconds):
     """The oracle's FP64 A-B pull step (the reference default pattern,
     defs.h:20-26), OpenMP collapse(2) over x,z as in state.hpp:1388-1397."""
@@ -368,6 +442,17 @@ def main():
                          ("f32", "tape")):
             adj[f"{st}_{mode}"] = adjoint_case(256, 96, 96, args.adjoint_steps, 200, storage=st, mode=mode,
                                                fd_dirs=args.adjoint_fd_dirs if st == "f64" and mode != "recompute" else 0)
+        # coupled scalar: throughput at 384^3 in FP16 storage, accuracy against FP64 at 192^3
+        adj["coupled_f16dev_384"] = coupled_adjoint_case(384, args.coupled_steps, "f16dev")
+        r64, g64 = coupled_adjoint_case(192, args.coupled_steps, "f64", fd_dirs=args.adjoint_fd_dirs // 8,
+                                        grad_out=True)
+        r16, g16 = coupled_adjoint_case(192, args.coupled_steps, "f16dev", grad_out=True)
+        import torch
+        adj["coupled_f64_192"] = r64
+        adj["coupled_f16dev_192"] = r16
+        adj["coupled_f16dev_192"]["grad_error_vs_fp64"] = {
+            "rel_L2": float(f"{float(torch.linalg.vector_norm(g16 - g64) / torch.linalg.vector_norm(g64)):.3e}"),
+            "rel_max": float(f"{float((g16 - g64).abs().max() / g64.abs().max()):.3e}")}
         out["adjoint"] = adj
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -181,3 +181,33 @@ def duct_flow_fields(X: int, Y: int, Z: int, umax: float):
     vx = np.broadcast_to(duct_profile(Y, Z, umax).T[None], (X, Y, Z)).copy()
     zero = np.zeros((X, Y, Z))
     return np.ones((X, Y, Z)), vx, zero, zero.copy()
+
+
+def gaussian_blobs(X: int, Y: int, Z: int, count: int, sigma: float, amplitude: float, seed: int,
+                   periodic=(True, True, False)) -> np.ndarray:
+    """Sum of `count` seeded Gaussian blobs amplitude * exp(-r^2 / (2 sigma^2))
+    with centres uniform in the box (periodic distance along the periodic
+    axes), each evaluated within 5 sigma of its centre.  Returns an [x, y, z]
+    float64 array."""
+    rng = np.random.default_rng(seed)
+    out = np.zeros((X, Y, Z))
+    dims = (X, Y, Z)
+    r = int(np.ceil(5 * sigma))
+    for _ in range(count):
+        c = rng.random(3) * np.array(dims)
+        idx, fac = [], []
+        for a in range(3):
+            n = dims[a]
+            lo, hi = int(np.floor(c[a])) - r, int(np.floor(c[a])) + r + 1
+            pts = np.arange(lo, hi)
+            if periodic[a]:
+                ix = np.mod(pts, n)
+            else:
+                keep = (pts >= 0) & (pts < n)
+                pts, ix = pts[keep], pts[keep]
+            d = pts - c[a]
+            idx.append(ix)
+            fac.append(np.exp(-d * d / (2 * sigma * sigma)))
+        blob = amplitude * fac[0][:, None, None] * fac[1][None, :, None] * fac[2][None, None, :]
+        np.add.at(out, np.ix_(idx[0], idx[1], idx[2]), blob)
+    return out
