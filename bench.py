@@ -3,11 +3,16 @@
 
 Headline workload (the value): FP64, 768^3 periodic box, seeded solenoidal
 turbulence (E(k) ~ k^4 exp(-2 (k/8)^2), u_rms = 0.04), tau = 0.5015, A-A
-pattern, default cumulant.  Per-precision lines (deviation storage FP32 /
-FP16 / BF16 and absolute FP32 at 512^3) and their rho/u error against an
-FP64 run of the same Taylor-Green vortex are reported beside it, as is the
-CPU oracle (a restatement of the reference's FP64 A-B host path) timed on a
-bounded sample on the host cores.
+pattern, default cumulant.  Beside it:
+- per-precision lines (absolute FP32 and deviation storage FP32 / FP16 /
+  BF16 / I16 at 512^3) with the rho/u error of each form against an FP64 run
+  after 2000 steps, for a Taylor-Green vortex and for the seeded turbulence,
+  both at 512^3, tau = 0.505;
+- the adjoint: a 256x96x96 duct over 2000 steps (inlet-profile gradient) and
+  the coupled NSE + passive scalar at 384^3 (source gradient), fwd+bwd
+  against forward throughput, with central-difference checks;
+- the CPU oracle (a restatement of the reference's FP64 A-B host path) timed
+  on a bounded sample on the host cores.
 
 Prints ONE JSON line.
 """
@@ -35,8 +40,8 @@ def parse():
     p.add_argument("--size", type=int, default=768, help="edge of the FP64 headline box")
     p.add_argument("--prec-size", type=int, default=512, help="edge of the per-precision boxes")
     p.add_argument("--prec-steps", type=int, default=100)
-    p.add_argument("--err-size", type=int, default=128, help="edge of the TGV used for the error report")
-    p.add_argument("--err-steps", type=int, default=500)
+    p.add_argument("--err-size", type=int, default=512, help="edge of the boxes of the error report")
+    p.add_argument("--err-steps", type=int, default=2000)
     p.add_argument("--no-precision", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
@@ -108,23 +113,45 @@ def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015
     return res
 
 
-def precision_errors(N, steps, storages):
-    """TGV (U0 = 0.05, tau = 0.505) run in FP64 and in each storage form;
-    L2 and max error of rho and u against the FP64 run after `steps` steps."""
+def precision_errors(N, steps, storages, ic="tgv"):
+    """The same initial condition run in FP64 and in each storage form; L2
+    and max error of rho and u against the FP64 run after `steps` steps.
+    ic "tgv": Taylor-Green vortex, U0 = 0.05; "turbulence": the seeded
+    spectrum of the headline box (u_rms = 0.04, k0 = 8).  tau = 0.505."""
     import torch
     from pytnl_lbm import D3Q27Lattice, tgv_fields
+    from pytnl_lbm.fields import turbulence_fields_torch
 
-    r = tgv_fields(N, N, N, 0.05)
+    from pytnl_lbm import i16_class_ranges
+
     nu = (0.505 - 0.5) / 3.0
+    if ic == "tgv":
+        u_max = 0.05
+    else:
+        u = turbulence_fields_torch(N, 1, 8.0, 0.04, device="cuda")
+        u_max = float(torch.sqrt(u[0] ** 2 + u[1] ** 2 + u[2] ** 2).max())
+        del u
+    # I16: one bound per direction class from the flow's velocity bound
+    Rc = i16_class_ranges(u_max)
+
+    def init(lat):
+        if ic == "tgv":
+            lat.set_equilibrium_field(*tgv_fields(N, N, N, 0.05))
+        else:
+            u = turbulence_fields_torch(N, 1, 8.0, 0.04, device="cuda")
+            lat.set_equilibrium_field(torch.ones_like(u[0]), *u)
+            del u
 
     def run(storage):
-        lat = D3Q27Lattice(N, N, N, storage=storage, nu=nu, store_range=0.5)
-        lat.set_equilibrium_field(*r)
+        lat = D3Q27Lattice(N, N, N, storage=storage, nu=nu, store_range=1.0, store_range_class=Rc)
+        init(lat)
+        torch.cuda.empty_cache()
         for t in range(steps):
             lat.step(output_macro=(t == steps - 1))
         torch.cuda.synchronize()
         out = [m.to(torch.float64).clone() for m in lat.macros()]
         del lat
+        torch.cuda.empty_cache()
         return out
 
     ref = run("f64")
@@ -132,16 +159,18 @@ def precision_errors(N, steps, storages):
     for st in storages:
         got = run(st)
         e = {}
-        for name, a, b in [("rho", got[0], ref[0])]:
-            d = a - b
-            e["rho_L2"] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(b - 1.0))
-            e["rho_max"] = float(d.abs().max() / (b - 1.0).abs().max())
+        d = got[0] - ref[0]
+        e["rho_L2"] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref[0] - 1.0))
+        e["rho_max"] = float(d.abs().max() / (ref[0] - 1.0).abs().max())
         du = torch.stack([got[i] - ref[i] for i in (1, 2, 3)])
         ub = torch.stack(ref[1:])
         e["u_L2"] = float(torch.linalg.vector_norm(du) / torch.linalg.vector_norm(ub))
         e["u_max"] = float(du.abs().max() / ub.abs().max())
         res[st] = e
-    return res, {"tgv": f"{N}^3", "U0": 0.05, "tau": 0.505, "steps": steps, "i16_store_range": 0.5,
+        del got, du, ub
+    return res, {"case": ic, "lattice": f"{N}^3", "tau": 0.505, "steps": steps,
+                 "i16_store_range_class": [round(r, 5) for r in Rc], "u_max": round(u_max, 5),
+                 "U0" if ic == "tgv" else "u_rms": 0.05 if ic == "tgv" else 0.04,
                  "rho_norm": "relative to rho-1 of the FP64 run"}
 
 
@@ -431,9 +460,13 @@ def main():
                                  "kernel_ms_avg": round(r["kernel_ms_avg"], 3),
                                  "kernel_ms_even": round(r["kernel_ms_even"], 3),
                                  "kernel_ms_odd": round(r["kernel_ms_odd"], 3)}
-        errs, spec = precision_errors(args.err_size, args.err_steps, ("f32", "f32dev", "f16dev", "bf16dev", "i16dev"))
-        for pst, e in errs.items():
-            per[f"{pst}_{P}"]["error_vs_fp64"] = {k: float(f"{v:.3e}") for k, v in e.items()}
+        spec = []
+        for ic in ("tgv", "turbulence"):
+            errs, sp = precision_errors(args.err_size, args.err_steps, ("f32", "f32dev", "f16dev", "bf16dev", "i16dev"),
+                                        ic=ic)
+            spec.append(sp)
+            for pst, e in errs.items():
+                per[f"{pst}_{P}"].setdefault("error_vs_fp64", {})[ic] = {k: float(f"{v:.3e}") for k, v in e.items()}
         out["per_precision"] = per
         out["precision_error_case"] = spec
     if not args.no_adjoint:
@@ -441,7 +474,7 @@ def main():
         for st, mode in (("f64", "segment"), ("f64", "tape"), ("f64", "recompute"), ("f32", "segment"),
                          ("f32", "tape")):
             adj[f"{st}_{mode}"] = adjoint_case(256, 96, 96, args.adjoint_steps, 200, storage=st, mode=mode,
-                                               fd_dirs=args.adjoint_fd_dirs if st == "f64" and mode != "recompute" else 0)
+                                               fd_dirs=args.adjoint_fd_dirs if (st, mode) == ("f64", "segment") else 0)
         # coupled scalar: throughput at 384^3 in FP16 storage, accuracy against FP64 at 192^3
         adj["coupled_f16dev_384"] = coupled_adjoint_case(384, args.coupled_steps, "f16dev")
         r64, g64 = coupled_adjoint_case(192, args.coupled_steps, "f64", fd_dirs=args.adjoint_fd_dirs // 8,

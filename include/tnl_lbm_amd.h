@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define TNL_LBM_ABI_VERSION 1u
+#define TNL_LBM_ABI_VERSION 2u
 
 /* ---- status codes ------------------------------------------------------ */
 #define TNL_LBM_OK 0
@@ -188,6 +188,11 @@ typedef struct tnl_lbm_d3q27_data {
 	 * Y*z + y; a non-NULL pointer overrides the constant component.        */
 	const double* inflow_profile[3];
 	double store_range;   /* I16_DEV only: R, the bound of |f - w_q| / w_q (> 0) */
+	/* I16_DEV only: the bound per direction class |c_q|^2 = 0, 1, 2, 3 (rest,
+	 * faces, edges, corners); a class with 0 uses store_range.  The classes'
+	 * deviations differ by orders of magnitude (the rest population's is
+	 * ~1.5|u|^2), so per-class bounds buy up to 6 bits of resolution.       */
+	double store_range_class[4];
 } tnl_lbm_d3q27_data;
 
 /* D3Q7 ADE data: ADE_Data (lbm_data.h:212-246) plus a source term that the

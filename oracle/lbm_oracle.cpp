@@ -170,8 +170,9 @@ int16_t f32_to_snorm16(float x)
 	const float c = x < -1.0f ? -1.0f : (x > 1.0f ? 1.0f : x);
 	return (int16_t) std::nearbyint(c * 32767.0f);
 }
-// bound R of |f - w_q| / w_q for the I16 deviation form (set per oracle call)
-double g_store_range = 1.0;
+// bound R_n of |f - w_q| / w_q per direction class |c_q|^2 = n for the I16
+// deviation form (set per oracle call: store_range_class[n], else store_range)
+double g_store_range[4] = {1.0, 1.0, 1.0, 1.0};
 
 float bf16_to_f32(uint16_t h)
 {
@@ -1102,14 +1103,15 @@ struct Store<TNL_LBM_STORE_I16_DEV>
 	using elem = int16_t;
 	static constexpr bool dev = true;
 	// decode: raw * (w_q * R/32767); encode: snorm16(v * (1/w_q) * (1/R))
+	static int cls(int q) { return (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0); }
 	static real load(const elem* p, int64_t i, int q)
 	{
-		const float Rs = (float) (g_store_range / 32767.0);
+		const float Rs = (float) (g_store_range[cls(q)] / 32767.0);
 		return (float) p[i] * ((float) w27(q) * Rs);
 	}
 	static void store(elem* p, int64_t i, int q, real v)
 	{
-		const float invR = (float) (1.0 / g_store_range);
+		const float invR = (float) (1.0 / g_store_range[cls(q)]);
 		p[i] = f32_to_snorm16(v * ((float) (1.0 / w27(q)) * invR));
 	}
 	static double to_abs(real v, int q) { return (double) v + w27(q); }
@@ -1777,9 +1779,12 @@ int check(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 		return fail(TNL_LBM_ERR_NULL, "population array missing");
 	if (cfg->storage < 0 || cfg->storage > 5 || cfg->pattern < 0 || cfg->pattern > 1)
 		return fail(TNL_LBM_ERR_ARG, "bad storage or pattern");
-	if (cfg->storage == TNL_LBM_STORE_I16_DEV && ! (SD->store_range > 0))
-		return fail(TNL_LBM_ERR_ARG, "store_range must be positive");
-	g_store_range = SD->store_range;
+	for (int n = 0; n < 4; n++) {
+		const double R = SD->store_range_class[n] > 0 ? SD->store_range_class[n] : SD->store_range;
+		if (cfg->storage == TNL_LBM_STORE_I16_DEV && ! (R > 0))
+			return fail(TNL_LBM_ERR_ARG, "store_range must be positive");
+		g_store_range[n] = R;
+	}
 	if (cfg->collision < TNL_LBM_COLL_CUM || cfg->collision > TNL_LBM_COLL_KBC_C4)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "unknown collision operator");
 	const bool dev = cfg->storage >= TNL_LBM_STORE_F32_DEV;

@@ -112,10 +112,13 @@ class HostLattice:
 
     def __init__(self, X, Y, Z, *, collision=_abi.COLL_CUM, storage=_abi.STORE_F64, pattern=_abi.PATTERN_AA,
                  macro_real=_abi.REAL_F64, nu=0.0, force=(0.0, 0.0, 0.0), geo=None, threads=None,
-                 store_range=1.0):
+                 store_range=1.0, store_range_class=None):
         self.cfg = _abi.Config.make(collision, storage, pattern, macro_real)
         self.d = _abi.D3Q27Data.make(X, Y, Z)
         self.d.store_range = store_range
+        if store_range_class is not None:
+            for n in range(4):
+                self.d.store_range_class[n] = store_range_class[n]
         self.X, self.Y, self.Z = X, Y, Z
         n = self.d.XYZ
         el = _NP_ELEM[storage]

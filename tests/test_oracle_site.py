@@ -194,3 +194,25 @@ def test_snorm16_conversion_is_round_to_nearest_even():
             got = lat.df[0].reshape(27)[0]
             want = np.int16(np.rint(np.float32(np.clip(np.float32(np.float32(f[0, 0, 0, 0] - W[0]) * np.float32((np.float32(1.0 / W[0])) * np.float32(1.0))), -1, 1)) * np.float32(32767.0)))
             assert got == want
+
+
+def test_i16_class_ranges_round_trip():
+    """With one bound R_n per direction class, an import/export round trip of
+    the fixed-point form is exact to half a quantum w_q R_n / 32767 of its
+    class, and a deviation beyond the class bound saturates at it."""
+    from lbm_common import CX, CY, CZ, W
+    Rc = (0.01, 0.2, 0.3, 0.4)
+    cls = np.array([int(c[0] != 0) + int(c[1] != 0) + int(c[2] != 0) for c in zip(CX, CY, CZ)])
+    R = np.asarray(Rc)[cls]
+    lat = O.HostLattice(4, 3, 2, storage=5, store_range=1.0, store_range_class=Rc)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, (27, 4, 3, 2)) * R[:, None, None, None] * 0.999
+    f = W[:, None, None, None] * (1 + x)
+    lat.import_f64(f)
+    back = lat.export_f64()
+    q = (W * R / 32767)[:, None, None, None]
+    assert (np.abs(back - f) <= 0.5 * q * 1.0001 + 1e-15).all()
+    f2 = W[:, None, None, None] * (1 + 2 * R[:, None, None, None] * np.sign(x))
+    lat.import_f64(f2)
+    sat = lat.export_f64() - W[:, None, None, None]
+    assert np.allclose(np.abs(sat), (W * R)[:, None, None, None], rtol=1e-6)

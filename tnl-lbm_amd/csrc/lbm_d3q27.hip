@@ -62,8 +62,13 @@ int validate(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD)
 		return fail(TNL_LBM_ERR_DIMS, "Y extent too large");
 	if (cfg->storage < TNL_LBM_STORE_F64 || cfg->storage > TNL_LBM_STORE_I16_DEV)
 		return fail(TNL_LBM_ERR_ARG, "unknown storage form");
-	if (cfg->storage == TNL_LBM_STORE_I16_DEV && ! (SD->store_range > 0))
-		return fail(TNL_LBM_ERR_ARG, "store_range must be positive for the fixed-point form");
+	if (cfg->storage == TNL_LBM_STORE_I16_DEV)
+		for (int n = 0; n < 4; n++) {
+			if (! (SD->store_range_class[n] >= 0))
+				return fail(TNL_LBM_ERR_ARG, "store_range_class must not be negative");
+			if (! (SD->store_range_class[n] > 0) && ! (SD->store_range > 0))
+				return fail(TNL_LBM_ERR_ARG, "store_range (or every store_range_class) must be positive for the fixed-point form");
+		}
 	if (cfg->pattern != TNL_LBM_PATTERN_AA && cfg->pattern != TNL_LBM_PATTERN_AB)
 		return fail(TNL_LBM_ERR_ARG, "unknown streaming pattern");
 	if (cfg->macro_real != TNL_LBM_REAL_F64 && cfg->macro_real != TNL_LBM_REAL_F32)
@@ -116,13 +121,24 @@ int dispatch_coll(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads
 	}
 
 // fixed-point scale factors (only the I16 form uses them)
-float rs_of(const tnl_lbm_d3q27_data* SD)
+// fixed-point bounds per direction class (store_range_class, else store_range)
+FixedRange range_of(const tnl_lbm_d3q27_data* SD)
 {
-	return (float) (SD->store_range / 32767.0);
+	FixedRange r;
+	for (int n = 0; n < 4; n++) {
+		const double R = SD->store_range_class[n] > 0 ? SD->store_range_class[n] : SD->store_range;
+		r.rs[n] = (float) (R / 32767.0);
+		r.invr[n] = R > 0 ? (float) (1.0 / R) : 0.0f;
+	}
+	return r;
 }
-float invr_of(const tnl_lbm_d3q27_data* SD)
+void set_range(const tnl_lbm_d3q27_data* SD, float* rs, float* invr)
 {
-	return (float) (1.0 / SD->store_range);
+	const FixedRange r = range_of(SD);
+	for (int n = 0; n < 4; n++) {
+		rs[n] = r.rs[n];
+		invr[n] = r.invr[n];
+	}
 }
 const int ELEM_BYTES[6] = {8, 4, 4, 2, 2, 2};
 
@@ -142,8 +158,7 @@ MacroArgs macro_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, co
 	M.ox = SD->ox; M.oy = SD->oy; M.oz = SD->oz;
 	M.Ys = SD->Y + 2 * SD->oy;
 	M.Zs = SD->Z + 2 * SD->oz;
-	M.rs = rs_of(SD);
-	M.invr = invr_of(SD);
+	set_range(SD, M.rs, M.invr);
 	return M;
 }
 
@@ -203,16 +218,15 @@ static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 	A.Ys = SD->Y + 2 * SD->oy;
 	A.Zs = SD->Z + 2 * SD->oz;
 	A.plane_bytes = (uint64_t) SD->XYZ * (uint64_t) ELEM_BYTES[cfg->storage];
-	A.rs = rs_of(SD);
-	A.invr = invr_of(SD);
+	set_range(SD, A.rs, A.invr);
 	{
 		const bool fast = cfg->storage != TNL_LBM_STORE_F64 && cfg->collision == TNL_LBM_COLL_CUM;
 		for (int n = 0; n < 4; n++) {
 			double w = 1.0;  // same product order as w27(): one value per class
 			for (int d = 0; d < 3; d++)
 				w *= d < n ? W1[0] : W1[1];
-			A.cls_sc[n] = (float) w * A.rs;
-			A.cls_inv[n] = (float) (1.0 / w) * A.invr * (fast ? 1.0f / (float) (1 << n) : 1.0f);
+			A.cls_sc[n] = (float) w * A.rs[n];
+			A.cls_inv[n] = (float) (1.0 / w) * A.invr[n] * (fast ? 1.0f / (float) (1 << n) : 1.0f);
 		}
 	}
 	A.offx = (int32_t) offset[0]; A.offy = (int32_t) offset[1]; A.offz = (int32_t) offset[2];
@@ -358,8 +372,7 @@ static int set_eq_impl(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, 
 	F.rho = rho; F.vx = vx; F.vy = vy; F.vz = vz;
 	for (int i = 0; i < 4; i++)
 		F.c[i] = c[i];
-	F.rs = rs_of(SD);
-	F.invr = invr_of(SD);
+	set_range(SD, F.rs, F.invr);
 	const unsigned grid = (unsigned) ((F.nsites + 255) / 256);
 	hipStream_t st = (hipStream_t) stream;
 	const int eqk = eq_kind_of(cfg->collision);
@@ -423,9 +436,9 @@ static int transfer(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, int
 	auto go = [&](auto stc) {
 		constexpr int ST = decltype(stc)::value;
 		if (exp)
-			hipLaunchKernelGGL((k_transfer<ST, 27, true>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n, rs_of(SD), invr_of(SD));
+			hipLaunchKernelGGL((k_transfer<ST, 27, true>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n, range_of(SD));
 		else
-			hipLaunchKernelGGL((k_transfer<ST, 27, false>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n, rs_of(SD), invr_of(SD));
+			hipLaunchKernelGGL((k_transfer<ST, 27, false>), dim3(grid), dim3(256), 0, st, SD->df[which], buf, SD->XYZ, n, range_of(SD));
 		return hip_check(hipGetLastError(), "transfer launch");
 	};
 	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));

@@ -195,14 +195,25 @@ struct Storage<TNL_LBM_STORE_I16_DEV>
 };
 template <>
 struct Bits<int16_t> { using type = uint16_t; };
-// per-direction scale factors of the fixed-point form (w_q R / 32767, 1/(w_q R))
-LBM_DEV float fx_sc(int q, float rs)
+// per-direction scale factors of the fixed-point form (w_q R_n / 32767, 1/(w_q R_n))
+// direction class |c_q|^2 (0 rest, 1 faces, 2 edges, 3 corners)
+constexpr int qclass(int q)
 {
-	return (float) w27(q) * rs;
+	return (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
 }
-LBM_DEV float fx_inv(int q, float invr)
+// the fixed-point bound R_n per class: rs = R_n / 32767 (decode), invr = 1 / R_n (encode)
+struct FixedRange
 {
-	return (float) (1.0 / w27(q)) * invr;
+	float rs[4];
+	float invr[4];
+};
+LBM_DEV float fx_sc(int q, const float* rs)
+{
+	return (float) w27(q) * rs[qclass(q)];
+}
+LBM_DEV float fx_inv(int q, const float* invr)
+{
+	return (float) (1.0 / w27(q)) * invr[qclass(q)];
 }
 
 // ---------------------------------------------------------------------------

@@ -30,7 +30,7 @@ struct StepArgs
 	int64_t ox, oy, oz;  // overlaps
 	int64_t Ys, Zs;      // stored extents Y+2oy, Z+2oz
 	uint64_t plane_bytes;  // XYZ * sizeof(element): byte stride between population planes
-	float rs, invr;        // fixed-point form: R / 32767 and 1 / R
+	float rs[4], invr[4];  // fixed-point form per direction class: R_n / 32767 and 1 / R_n
 	// fixed-point form per weight class n = |c_q|^2 (w_n = (2/3)^(3-n) / 6^n):
 	// decode w_n R / 32767, encode 1 / (w_n R) times the fast collision's
 	// 2^-n output factor
@@ -820,7 +820,7 @@ struct FieldArgs
 	const double* vy;
 	const double* vz;
 	double c[4];
-	float rs, invr;
+	float rs[4], invr[4];
 };
 
 template <int ST, int EQK>
@@ -849,7 +849,7 @@ __global__ void k_set_equilibrium(const FieldArgs A)
 
 struct MacroArgs
 {
-	float rs, invr;
+	float rs[4], invr[4];
 	const void* df;
 	void* macro;
 	int macro_f64;
@@ -885,7 +885,7 @@ __global__ void k_compute_macro(const MacroArgs A)
 }
 
 template <int ST, int Q, bool EXPORT>
-__global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites, float rs, float invr)
+__global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites, const FixedRange R)
 {
 	using S = Storage<ST>;
 	const int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -896,9 +896,9 @@ __global__ void k_transfer(void* df, double* buf, int64_t XYZ, int64_t nsites, f
 	for (int q = 0; q < Q; q++) {
 		const double w = S::dev ? (Q == 27 ? w27(q) : (q == 0 ? 0.25 : 0.125)) : 0.0;
 		if constexpr (EXPORT)
-			buf[q * XYZ + s] = (double) S::ld(p + q * XYZ + s, fx_sc(q, rs)) + w;
+			buf[q * XYZ + s] = (double) S::ld(p + q * XYZ + s, fx_sc(q, R.rs)) + w;
 		else
-			S::st(p + q * XYZ + s, (typename S::real) (buf[q * XYZ + s] - w), fx_inv(q, invr));
+			S::st(p + q * XYZ + s, (typename S::real) (buf[q * XYZ + s] - w), fx_inv(q, R.invr));
 	}
 }
 

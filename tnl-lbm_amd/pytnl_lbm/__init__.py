@@ -8,13 +8,13 @@ nothing falls back to the CPU.
 """
 from . import _abi
 from ._abi import ADE_GEO, GEO, LBMError
-from .lattice import D3Q7Lattice, D3Q27Lattice, from_storage, to_storage
+from .lattice import D3Q7Lattice, D3Q27Lattice, from_storage, i16_class_ranges, to_storage
 from .fields import (channel_geometry, duct_flow_fields, duct_profile, gaussian_blobs, inflow_control, smooth_perturbation,
                      tgv_fields, turbulence_fields)
 from .adjoint import (CheckpointedRun, CoupledScalarRun, density_velocity, lattice_run, populations, scalar_run,
                       top_layer_variance)
 
-__all__ = ["_abi", "GEO", "ADE_GEO", "LBMError", "D3Q27Lattice", "D3Q7Lattice", "to_storage", "from_storage",
+__all__ = ["_abi", "GEO", "ADE_GEO", "LBMError", "D3Q27Lattice", "D3Q7Lattice", "to_storage", "from_storage", "i16_class_ranges",
            "tgv_fields", "turbulence_fields", "gaussian_blobs", "channel_geometry", "duct_flow_fields", "duct_profile", "inflow_control",
            "smooth_perturbation", "CheckpointedRun", "density_velocity", "lattice_run", "populations", "CoupledScalarRun",
            "scalar_run", "top_layer_variance"]

@@ -10,7 +10,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # status codes
 OK = 0
@@ -122,7 +122,8 @@ class D3Q27Data(C.Structure):
                 ("lbm_viscosity", C.c_double), ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double),
                 ("df", C.c_void_p * 2), ("macro", C.c_void_p), ("map", C.c_void_p),
                 ("inflow_vx", C.c_double), ("inflow_vy", C.c_double), ("inflow_vz", C.c_double),
-                ("inflow_profile", C.c_void_p * 3), ("store_range", C.c_double)]
+                ("inflow_profile", C.c_void_p * 3), ("store_range", C.c_double),
+                ("store_range_class", C.c_double * 4)]
 
     @classmethod
     def make(cls, X, Y, Z, ox=0, oy=0, oz=0, XYZ=None):

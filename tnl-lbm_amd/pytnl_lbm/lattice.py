@@ -31,6 +31,17 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def i16_class_ranges(u_max: float, margin: float = 1.25):
+    """Bounds of |f - w_q| / w_q per direction class |c_q|^2 = n for a flow
+    with |u| <= u_max near rho = 1, for the I16 deviation form: the
+    equilibrium's deviation 3 c.u + 4.5 (c.u)^2 - 1.5 u^2 with |c.u| <=
+    sqrt(n) u_max, a density excursion 3 u_max^2 on top, times a margin for
+    the non-equilibrium part."""
+    d = 3.0 * u_max * u_max
+    return tuple(margin * ((1 + d) * (3 * n ** 0.5 * u_max + 4.5 * n * u_max ** 2 + 1.5 * u_max ** 2) + d)
+                 for n in range(4))
+
+
 def to_storage(a: torch.Tensor) -> torch.Tensor:
     """[x, y, z] array -> contiguous storage order (x, z, y)."""
     return a.permute(0, 2, 1).contiguous()
@@ -48,7 +59,7 @@ class D3Q27Lattice:
 
     def __init__(self, X: int, Y: int, Z: int, *, collision="CUM", storage="f64", pattern="AA",
                  macro_real=None, nu: float = 0.0, force=(0.0, 0.0, 0.0), geo=None, device="cuda:0",
-                 XYZ: int | None = None, store_range: float = 1.0):
+                 XYZ: int | None = None, store_range: float = 1.0, store_range_class=None):
         if not torch.cuda.is_available():
             raise RuntimeError("D3Q27Lattice needs a GPU (libtnl_lbm_amd.so runs on gfx950 only)")
         self.lib = _abi.load_library()
@@ -60,6 +71,9 @@ class D3Q27Lattice:
         self.cfg = _abi.Config.make(coll, st, pat, macro_real)
         self.d = _abi.D3Q27Data.make(X, Y, Z, XYZ=XYZ)
         self.d.store_range = store_range
+        if store_range_class is not None:
+            for n in range(4):
+                self.d.store_range_class[n] = store_range_class[n]
         self.X, self.Y, self.Z = X, Y, Z
         self.device = torch.device(device)
         n = self.d.XYZ
