@@ -406,6 +406,7 @@ struct General
 	using P = Phys<ST>;
 	static constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
 	static constexpr int EQK = eq_kind_of(COLL);
+	static constexpr bool FAST = fast_collision<ST, COLL>;
 
 	static LBM_DEV int64_t sidx(const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
 	{
@@ -624,13 +625,29 @@ struct General
 						break;
 					}
 				default:
-					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+					// the fast FP32 collision sums its own moments at plain collided sites
+					if (! (FAST && (gi == GEO_FLUID || gi == GEO_PERIODIC)))
+						P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
 					break;
 			}
 			// doCollision (bc.h:406-412)
 			const bool do_coll = gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_RIGHT_INTERP
 							  || gi == GEO_INFLOW_LEFT || gi == GEO_INFLOW_BB_LEFT;
-			if (do_coll)
+			if constexpr (FAST) {
+				// FP32 forms, default cumulant: the raw-moment formulation (as the periodic kernel)
+				// wherever rho and u are the site's own; the boundary codes that replace them keep
+				// the reference chain
+				if (gi == GEO_FLUID || gi == GEO_PERIODIC) {
+					using FC = FastCum<S::dev>;
+					FC::collide(f, Fx, Fy, Fz, (real) A.omega1, true, rho, vx, vy, vz);
+#pragma unroll
+					for (int q = 0; q < 27; q++)
+						f[q] *= FC::unscale(q);
+				}
+				else if (do_coll)
+					collide<ST, COLL>(f, rho, vx, vy, vz, A);
+			}
+			else if (do_coll)
 				collide<ST, COLL>(f, rho, vx, vy, vz, A);
 			// postCollision streaming
 			if (AA && A.even) {
