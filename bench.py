@@ -186,6 +186,7 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
     from pytnl_lbm import (CheckpointedRun, D3Q27Lattice, _abi, channel_geometry, density_velocity,
                            duct_flow_fields, inflow_control, lattice_run, populations)
 
+    torch.cuda.empty_cache()
     lat = D3Q27Lattice(X, Y, Z, storage=storage, pattern="AB", nu=(0.51 - 0.5) / 3, geo=channel_geometry(X, Y, Z))
     lat.set_equilibrium_field(*duct_flow_fields(X, Y, Z, 0.04))
     f0 = lat.df[0].clone()
@@ -226,7 +227,7 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
     lam = [torch.randn(f0.numel(), device="cuda", dtype=f0.dtype) * 1e-3 for _ in range(2)]
     grad = torch.zeros(3, Z, Y, dtype=torch.float64, device="cuda")
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(adj_reps)]
-    tape = None if run.tape is None else run.tape[0]
+    tape = run.rec[0] if run.rec is not None else (None if run.tape is None else run.tape[0])
     for i in range(adj_reps + 2):
         if i >= 2:
             ev[i - 2][0].record()
@@ -244,12 +245,18 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
     res = {
         "workload": f"duct {X}x{Y}x{Z}, tau=0.51, T={T}, J on plane x={xp}, {storage.upper()} A-B cumulant",
         "schedule": mode if tape is None else f"{mode} ({'float64' if run.tape_f64 else 'float32'} moment recording)",
+        "schedule_note": {"hybrid": "sqrt(T) checkpoints; the forward records the moments of the trailing segments "
+                                    "that fit in 80% of free HBM, only the rest is recomputed",
+                          "segment": "sqrt(T) checkpoints; each segment recomputed with recording",
+                          "recompute": "sqrt(T) checkpoints; each segment's states recomputed and kept",
+                          "tape": "every step recorded, no checkpoints"}[mode],
         "forward_mlups": round(sites * T / t_fwd / 1e6, 1),
         "fwd_bwd_mlups": round(sites * T / t_fb / 1e6, 1),
         "ratio": round(t_fwd / t_fb, 4),
         "ratio_definition": "fwd_bwd_mlups / forward_mlups; forward = the plain A-B step, T steps",
         "checkpoint_stride": run.K if mode != "tape" else None,
         "checkpoints": run.nseg if mode != "tape" else 0,
+        "recorded_steps": run.recorded_steps if mode == "hybrid" else (T if mode == "tape" else 0),
         "J": Jv.item(),
         "adjoint_kernel": {"ms_avg": round(adj_s * 1e3, 3), "bytes_per_site": adj_site,
                            "achieved_GBs": round(adj_bytes / adj_s / 1e9, 1),
@@ -267,7 +274,7 @@ def adjoint_case(X, Y, Z, T, xp, storage="f64", fd_dirs=16, eps=1e-4, adj_reps=2
                 fd = (J(p + eps * d) - J(p - eps * d)).item() / (2 * eps)
                 worst = max(worst, abs((g * d).sum().item() - fd) / abs(fd))
         res["fd_check"] = {"eps": eps, "directions": fd_dirs, "max_rel_err": float(f"{worst:.3e}")}
-    del run, lat
+    del run, lat, J, Jv, pr, g, lam, grad, tape
     torch.cuda.empty_cache()
     return res
 
@@ -471,10 +478,10 @@ def main():
         out["precision_error_case"] = spec
     if not args.no_adjoint:
         adj = {}
-        for st, mode in (("f64", "segment"), ("f64", "tape"), ("f64", "recompute"), ("f32", "segment"),
-                         ("f32", "tape")):
+        for st, mode in (("f64", "hybrid"), ("f64", "segment"), ("f64", "recompute"), ("f64", "tape"),
+                         ("f32", "hybrid"), ("f32", "segment")):
             adj[f"{st}_{mode}"] = adjoint_case(256, 96, 96, args.adjoint_steps, 200, storage=st, mode=mode,
-                                               fd_dirs=args.adjoint_fd_dirs if (st, mode) == ("f64", "segment") else 0)
+                                               fd_dirs=args.adjoint_fd_dirs if (st, mode) == ("f64", "hybrid") else 0)
         # coupled scalar: throughput at 384^3 in FP16 storage, accuracy against FP64 at 192^3
         adj["coupled_f16dev_384"] = coupled_adjoint_case(384, args.coupled_steps, "f16dev")
         r64, g64 = coupled_adjoint_case(192, args.coupled_steps, "f64", fd_dirs=args.adjoint_fd_dirs // 8,

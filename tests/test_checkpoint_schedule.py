@@ -37,9 +37,9 @@ def quad(f):
 
 
 class FakeRun(CheckpointedRun):
-    def __init__(self, T, K, A, B, mode):
+    def __init__(self, T, K, A, B, mode, budget=None):
         self.A, self.B, self.nsteps, self.nadj = A, B, 0, 0
-        super().__init__(fake_lattice(), T, K, mode=mode, tape_f64=True)
+        super().__init__(fake_lattice(), T, K, mode=mode, tape_f64=True, record_budget=budget)
 
     def _step(self, src, dst, tape=None, stream=None):
         assert src.data_ptr() != dst.data_ptr()
@@ -59,9 +59,13 @@ class FakeRun(CheckpointedRun):
         self.nadj += 1
 
 
-@pytest.mark.parametrize("mode", ["recompute", "segment", "tape"])
+STEP_BYTES = W * 8  # one step's float64 recording of the fake lattice
+
+
+@pytest.mark.parametrize("mode,budget", [("recompute", None), ("segment", None), ("tape", None),
+                                         ("hybrid", 0), ("hybrid", 12 * STEP_BYTES), ("hybrid", 1000 * STEP_BYTES)])
 @pytest.mark.parametrize("T,K", [(1, None), (7, 1), (7, 7), (16, None), (17, None), (23, 5)])
-def test_gradient_equals_unrolled_autograd(mode, T, K):
+def test_gradient_equals_unrolled_autograd(mode, budget, T, K):
     g = torch.Generator().manual_seed(T * 31 + (K or 0))
     # a contraction, so that 23 steps of the quadratic map stay bounded
     A = 0.7 * torch.eye(N, dtype=torch.float64) + 0.2 / N**0.5 * torch.randn(N, N, generator=g, dtype=torch.float64)
@@ -79,7 +83,7 @@ def test_gradient_equals_unrolled_autograd(mode, T, K):
         f = A @ f + B @ pr.reshape(-1) + quad(f)
     loss(f).backward()
 
-    run = FakeRun(T, K, A, B, mode)
+    run = FakeRun(T, K, A, B, mode, budget)
     pc, fc = p.clone().requires_grad_(), f0.clone().requires_grad_()
     fT = lattice_run(run, pc, fc)
     assert torch.allclose(fT.detach(), f.detach(), rtol=1e-14, atol=1e-14)
@@ -91,8 +95,15 @@ def test_gradient_equals_unrolled_autograd(mode, T, K):
         assert recomputed == 0 and run.tape.shape[0] == T and not run.ckpt
     elif mode == "recompute":
         assert recomputed == T - run.nseg and len(run.seg) == run.K - 1
-    else:  # every step of every segment is recorded once more
+    elif mode == "segment":  # every step of every segment is recorded once more
         assert recomputed == T and run.tape.shape[0] == run.K
+    else:  # the recorded trailing segments are not recomputed, the others are, with recording
+        assert recomputed == T - run.recorded_steps
+        assert run.recorded_steps == sum(run._seg_len(j) for j in range(run.rec_first, run.nseg))
+        if budget >= 1000 * STEP_BYTES:
+            assert run.recorded_steps == T
+        if budget == 0:
+            assert run.recorded_steps == 0
     np.testing.assert_allclose(pc.grad.numpy(), pr.grad.numpy(), rtol=1e-11, atol=1e-11 * pr.grad.abs().max().item())
     np.testing.assert_allclose(fc.grad.numpy(), fr.grad.numpy(), rtol=1e-11, atol=1e-11 * fr.grad.abs().max().item())
 

@@ -107,7 +107,7 @@ def duct_problem(X=32, Y=16, Z=16, T=120, xp=24, storage="f64", K=None, mode="re
     return lat, run, p, f0, J
 
 
-@pytest.mark.parametrize("mode,tape_f64", [("recompute", None), ("segment", None), ("tape", False)])
+@pytest.mark.parametrize("mode,tape_f64", [("recompute", None), ("segment", None), ("tape", False), ("hybrid", False)])
 def test_inflow_gradient_matches_central_differences(mode, tape_f64):
     """The duct with a perturbed inlet profile, objective 1/2 sum |u - u_target|^2
     on one x plane after T steps: dJ/d(profile) . d against
@@ -145,7 +145,7 @@ def test_gradient_does_not_depend_on_the_schedule():
     within 1e-6."""
     grads = {}
     for key in [("recompute", 1, None), ("recompute", None, None), ("recompute", 120, None), ("segment", None, None),
-                ("segment", 7, None), ("tape", None, False), ("tape", None, True)]:
+                ("segment", 7, None), ("tape", None, False), ("tape", None, True), ("hybrid", None, True)]:
         _, _, p, f0, J = duct_problem(T=120, K=key[1], mode=key[0], tape_f64=key[2])
         pr = p.clone().requires_grad_()
         J(pr, f0).backward()

@@ -48,7 +48,7 @@ def density_velocity(f: torch.Tensor, force=(0.0, 0.0, 0.0)):
     return rho, (j + 0.5 * F) / rho
 
 
-MODES = ("recompute", "segment", "tape")
+MODES = ("recompute", "segment", "tape", "hybrid")
 
 
 def tape_elements(XYZ: int) -> int:
@@ -80,6 +80,12 @@ class CheckpointedRun:
     - ``"tape"``: every forward step records its moments; the backward pass
       reads them and recomputes nothing.  Memory: T recordings (float32 by
       default: 40 bytes per site and step).
+    - ``"hybrid"``: the checkpoints of ``"segment"``, and the forward pass
+      also records the moments of as many trailing segments as
+      ``record_budget`` bytes hold (default: 80% of the device memory free
+      once the other buffers exist); only the segments before those are
+      recomputed backwards.  With enough memory nothing is recomputed; with
+      none it is ``"segment"``.  Float recordings unless ``tape_f64``.
 
     Recordings are float64 for F64 storage in ``"segment"`` mode (the
     gradient then equals the ``"recompute"`` one) and float32 otherwise,
@@ -89,7 +95,7 @@ class CheckpointedRun:
     CUM."""
 
     def __init__(self, lat: D3Q27Lattice, T: int, K: int | None = None, mode: str = "segment",
-                 tape_f64: bool | None = None):
+                 tape_f64: bool | None = None, record_budget: int | None = None):
         if lat.cfg.pattern != _abi.PATTERN_AB:
             raise ValueError("the adjoint differentiates the A-B step: build the lattice with pattern='AB'")
         if lat.cfg.collision != _abi.COLL_CUM or lat.cfg.storage not in (_abi.STORE_F64, _abi.STORE_F32):
@@ -115,14 +121,51 @@ class CheckpointedRun:
         self.ckpt = [z() for _ in range(self.nseg)] if mode != "tape" else []
         self.pp = [z() for _ in range(2)]
         self.seg = [z() for _ in range(self.K - 1)] if mode == "recompute" else []
-        ntape = {"recompute": 0, "segment": self.K, "tape": T}[mode]
+        ntape = {"recompute": 0, "segment": self.K, "tape": T, "hybrid": 0}[mode]
         self.tape = torch.zeros(ntape, tape_elements(lat.d.XYZ), dtype=tdt, device=dev) if ntape else None
         self.lam = [z() for _ in range(2)]
+        self.rec_first, self.rec, self.rec_offset = self.nseg, None, {}
+        if mode == "hybrid":
+            self._plan_recording(tdt, record_budget)
         self.profile = torch.zeros(3, lat.Z, lat.Y, dtype=torch.float64, device=dev)
         lat.set_inflow(0.0, 0.0, 0.0, profile=[self.profile[0], self.profile[1], self.profile[2]])
         for k in range(3):  # bind the control buffer itself
             lat._profiles[k] = self.profile[k]
             lat.d.inflow_profile[k] = self.profile[k].data_ptr()
+
+    def _seg_len(self, j):
+        return min(self.T, (j + 1) * self.K) - j * self.K
+
+    def _plan_recording(self, tdt, budget):
+        """Record the trailing segments that fit in `budget` bytes, keeping room
+        for one segment's recording for the recomputed ones."""
+        lat = self.lat
+        per_step = tape_elements(lat.d.XYZ) * torch.empty((), dtype=tdt).element_size()
+        if budget is None:
+            torch.cuda.empty_cache()  # blocks torch caches but no longer uses count as free here
+            budget = int(0.8 * torch.cuda.mem_get_info(lat.device)[0])
+        steps = budget // per_step
+        nrec, total = 0, 0
+        while nrec < self.nseg:
+            m = self._seg_len(self.nseg - 1 - nrec)
+            reserve = 0 if nrec + 1 == self.nseg else self.K
+            if total + m + reserve > steps:
+                break
+            total += m
+            nrec += 1
+        self.rec_first = self.nseg - nrec
+        off = 0
+        for j in range(self.rec_first, self.nseg):
+            self.rec_offset[j] = off
+            off += self._seg_len(j)
+        el = tape_elements(lat.d.XYZ)
+        self.rec = torch.zeros(total, el, dtype=tdt, device=lat.device) if total else None
+        self.tape = torch.zeros(self.K, el, dtype=tdt, device=lat.device) if self.rec_first > 0 else None
+
+    @property
+    def recorded_steps(self) -> int:
+        """Steps whose moments the forward pass records (the others are recomputed)."""
+        return 0 if self.rec is None else self.rec.shape[0]
 
     # ---- single launches ------------------------------------------------------
     def _step(self, src: torch.Tensor, dst: torch.Tensor, tape: torch.Tensor | None = None, stream=None):
@@ -171,7 +214,9 @@ class CheckpointedRun:
             for t in range(self.T):
                 nxt = t + 1
                 dst = self.ckpt[nxt // self.K] if nxt % self.K == 0 and nxt < self.T else self._other(src)
-                self._step(src, dst)
+                j = t // self.K
+                tape = self.rec[self.rec_offset[j] + t - j * self.K] if j >= self.rec_first else None
+                self._step(src, dst, tape)
                 src = dst
             return src
         finally:
@@ -192,7 +237,11 @@ class CheckpointedRun:
             for j in reversed(range(self.nseg)):
                 t0 = j * self.K
                 m = min(self.T, t0 + self.K) - t0
-                if self.mode == "recompute":
+                if j >= self.rec_first:
+                    for i in reversed(range(m)):
+                        self._adjoint(None, cur, other, grad, self.rec[self.rec_offset[j] + i])
+                        cur, other = other, cur
+                elif self.mode == "recompute":
                     states = [self.ckpt[j]] + self.seg[:m - 1]
                     for i in range(m - 1):
                         self._step(states[i], states[i + 1])
