@@ -1,0 +1,207 @@
+/* C++ face of libtnl_lbm_amd for code built on the reference's plugin API.
+ *
+ * The reference selects its lattice update at compile time through
+ * LBM_CONFIG<TRAITS, KernelStruct, DATA, COLL, EQ, STREAMING, BC, MACRO>
+ * (lbm3d/defs.h:288-310) and launches cudaLBMKernel<NSE> per block
+ * (lbm3d/state.hpp:1302-1309).  This header maps those same plugin types
+ * onto the run-time tnl_lbm_config of the C ABI, so that a reference-style
+ * State::SimUpdate keeps its types and replaces only the launch:
+ *
+ *     #include <tnl_lbm_amd.hpp>
+ *     ...
+ *     tnl_lbm_amd::step<NSE>(block.data, offset, end, block.is_distributed(), compute_macro, stream);
+ *
+ * Mapped types (forward-declared below with the reference's own template
+ * parameter lists, lbm3d/d3q27/col_*.h; declared without default arguments so
+ * that the reference's headers may come before or after this one):
+ *   D3Q27_CUM<T, EQ>                 -> TNL_LBM_COLL_CUM, or CUM_2017 /
+ *                                       CUM_2017_ANTIALIAS under the
+ *                                       reference's USE_GEIER_CUM_2017 /
+ *                                       USE_GEIER_CUM_ANTIALIAS (defs.h:314-315)
+ *   D3Q27_MRT<T, EQ>                 -> TNL_LBM_COLL_MRT_LES
+ *   D3Q27_KBC_N1..N4, D3Q27_KBC_C1..C4 -> TNL_LBM_COLL_KBC_*
+ *   D3Q27_STREAMING                  -> A-A under AA_PATTERN, else A-B
+ *                                       (core.h:20-25)
+ *   TRAITS::dreal double / float     -> F64 / F32 storage and macro arrays
+ * The deviation storage forms (F32_DEV ... I16_DEV) have no reference type:
+ * pass the storage explicitly.  An unsupported collision fails to compile.
+ */
+#pragma once
+
+#include <cstdint>
+#include <type_traits>
+
+#include "tnl_lbm_amd.h"
+
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_CUM;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_MRT;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_N1;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_N2;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_N3;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_N4;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_C1;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_C2;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_C3;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q27_KBC_C4;
+
+namespace tnl_lbm_amd {
+
+template <typename COLL>
+struct collision_id
+{
+	static_assert(sizeof(COLL) == 0, "this collision operator has no counterpart in libtnl_lbm_amd");
+};
+
+template <typename T, typename E>
+struct collision_id<D3Q27_CUM<T, E>>
+{
+#if defined(USE_GEIER_CUM_2017) && defined(USE_GEIER_CUM_ANTIALIAS)
+	static constexpr int value = TNL_LBM_COLL_CUM_2017_ANTIALIAS;
+#elif defined(USE_GEIER_CUM_2017)
+	static constexpr int value = TNL_LBM_COLL_CUM_2017;
+#elif defined(USE_GEIER_CUM_ANTIALIAS)
+	static_assert(sizeof(T) == 0, "the antialias correction is built together with the 2017 limiter only");
+#else
+	static constexpr int value = TNL_LBM_COLL_CUM;
+#endif
+};
+// clang-format off
+template <typename T, typename E> struct collision_id<D3Q27_MRT<T, E>>    { static constexpr int value = TNL_LBM_COLL_MRT_LES; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_N1<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_N1; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_N2<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_N2; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_N3<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_N3; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_N4<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_N4; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_C1<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_C1; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_C2<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_C2; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_C3<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_C3; };
+template <typename T, typename E> struct collision_id<D3Q27_KBC_C4<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_C4; };
+// clang-format on
+
+// the streaming plugin the reference compiled in (core.h:20-25)
+#ifdef AA_PATTERN
+constexpr int pattern_id = TNL_LBM_PATTERN_AA;
+#else
+constexpr int pattern_id = TNL_LBM_PATTERN_AB;
+#endif
+
+template <typename DREAL>
+constexpr int storage_of()
+{
+	static_assert(std::is_same<DREAL, double>::value || std::is_same<DREAL, float>::value, "dreal must be double or float");
+	return std::is_same<DREAL, double>::value ? TNL_LBM_STORE_F64 : TNL_LBM_STORE_F32;
+}
+
+// tnl_lbm_config of an LBM_CONFIG bundle (NSE::COLL, NSE::TRAITS::dreal); storage < 0: the dreal form
+template <typename NSE>
+inline tnl_lbm_config config_for(int storage = -1)
+{
+	using dreal = typename NSE::TRAITS::dreal;
+	tnl_lbm_config c{};
+	c.struct_size = sizeof c;
+	c.version = TNL_LBM_ABI_VERSION;
+	c.collision = collision_id<typename NSE::COLL>::value;
+	c.storage = storage < 0 ? storage_of<dreal>() : storage;
+	c.pattern = pattern_id;
+	c.macro_real = std::is_same<dreal, double>::value ? TNL_LBM_REAL_F64 : TNL_LBM_REAL_F32;
+	return c;
+}
+
+namespace detail {
+template <typename D, typename = void>
+struct has_inflow : std::false_type
+{};
+template <typename D>
+struct has_inflow<D, std::void_t<decltype(std::declval<D&>().inflow_vx)>> : std::true_type
+{};
+template <typename D, typename = void>
+struct has_force : std::false_type
+{};
+template <typename D>
+struct has_force<D, std::void_t<decltype(std::declval<D&>().fx)>> : std::true_type
+{};
+}  // namespace detail
+
+// tnl_lbm_d3q27_data of a block's NSE::DATA (LBM_Data, NSE_Data,
+// NSE_Data_ConstInflow or NSE_Data_InflowProfile, lbm_data.h:8-133)
+template <typename DATA>
+inline tnl_lbm_d3q27_data data_for(DATA& D)
+{
+	tnl_lbm_d3q27_data sd{};
+	sd.struct_size = sizeof sd;
+	sd.version = TNL_LBM_ABI_VERSION;
+	sd.X = D.X();
+	sd.Y = D.Y();
+	sd.Z = D.Z();
+	sd.ox = D.indexer.template getOverlap<0>();  // as kernels.h:39-41 reads them
+	sd.oy = D.indexer.template getOverlap<1>();
+	sd.oz = D.indexer.template getOverlap<2>();
+	sd.XYZ = D.XYZ;
+	sd.even_iter = D.even_iter ? 1 : 0;
+	sd.lbm_viscosity = D.lbmViscosity;
+	if constexpr (detail::has_force<DATA>::value) {
+		sd.fx = D.fx;
+		sd.fy = D.fy;
+		sd.fz = D.fz;
+	}
+	sd.df[0] = D.dfs[0];
+	sd.df[1] = D.dfs[1];
+	sd.macro = D.dmacro;
+	sd.map = D.dmap;
+	if constexpr (detail::has_inflow<DATA>::value) {
+		using V = std::remove_reference_t<decltype(D.inflow_vx)>;
+		if constexpr (std::is_pointer<V>::value) {
+			// NSE_Data_InflowProfile: per-(y, z) arrays indexed Y*z + y
+			static_assert(std::is_same<std::remove_cv_t<std::remove_pointer_t<V>>, double>::value,
+						  "inflow profiles are passed as double arrays");
+			sd.inflow_profile[0] = D.inflow_vx;
+			sd.inflow_profile[1] = D.inflow_vy;
+			sd.inflow_profile[2] = D.inflow_vz;
+		}
+		else {
+			sd.inflow_vx = D.inflow_vx;
+			sd.inflow_vy = D.inflow_vy;
+			sd.inflow_vz = D.inflow_vz;
+		}
+	}
+	return sd;
+}
+
+// flags of the reference's call: distributed (x, y, z) and output_macro
+template <typename DIST>
+inline uint32_t flags_for(const DIST& distributed, bool output_macro)
+{
+	uint32_t f = output_macro ? TNL_LBM_FLAG_OUTPUT_MACRO : 0u;
+	if (distributed.x())
+		f |= TNL_LBM_FLAG_DISTRIBUTED_X;
+	if (distributed.y())
+		f |= TNL_LBM_FLAG_DISTRIBUTED_Y;
+	if (distributed.z())
+		f |= TNL_LBM_FLAG_DISTRIBUTED_Z;
+	return f;
+}
+
+// One launch of cudaLBMKernel<NSE>(SD, offset, end, distributed, output_macro)
+// (kernels.h:59-112): returns the library's status (0 = ok, negative =
+// error; tnl_lbm_last_error() says which).  IDX3D: anything with x(), y(), z().
+template <typename NSE, typename DATA, typename IDX3D, typename DIST>
+inline int step(DATA& SD, const IDX3D& offset, const IDX3D& end, const DIST& distributed, bool output_macro,
+				void* stream = nullptr, int storage = -1)
+{
+	const tnl_lbm_config cfg = config_for<NSE>(storage);
+	const tnl_lbm_d3q27_data sd = data_for(SD);
+	const int64_t b[3] = {(int64_t) offset.x(), (int64_t) offset.y(), (int64_t) offset.z()};
+	const int64_t e[3] = {(int64_t) end.x(), (int64_t) end.y(), (int64_t) end.z()};
+	return tnl_lbm_d3q27_step(&cfg, &sd, b, e, flags_for(distributed, output_macro), stream);
+}
+
+}  // namespace tnl_lbm_amd

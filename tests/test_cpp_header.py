@@ -1,0 +1,113 @@
+"""include/tnl_lbm_amd.hpp maps the reference's plugin types onto the C ABI.
+Compiled here with g++ against stand-ins that carry the reference's own
+template parameter lists (lbm3d/d3q27/col_*.h, declared after the header,
+with their default arguments) and LBM_Data / NSE_Data field names
+(lbm_data.h:8-133): the configuration and data struct it builds are checked
+for each compile-time switch the reference uses (USE_GEIER_CUM_2017,
+USE_GEIER_CUM_ANTIALIAS, AA_PATTERN), and an operator without a counterpart
+must fail to compile."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+LIBDIR = ROOT / "tnl-lbm_amd" / "pytnl_lbm"
+
+SRC = r'''
+#include <tnl_lbm_amd.hpp>
+#include <cstdio>
+
+template <typename TRAITS> struct D3Q27_EQ {};
+template <typename TRAITS> struct D3Q27_EQ_ENTROPIC {};
+template <typename TRAITS, typename LBM_EQ = D3Q27_EQ<TRAITS>> struct D3Q27_CUM {};
+template <typename TRAITS, typename LBM_EQ = D3Q27_EQ<TRAITS>> struct D3Q27_MRT {};
+template <typename TRAITS, typename LBM_EQ = D3Q27_EQ_ENTROPIC<TRAITS>> struct D3Q27_KBC_N4 {};
+template <typename TRAITS, typename LBM_EQ = D3Q27_EQ<TRAITS>> struct D3Q27_KBC_C1 {};
+template <typename TRAITS, typename LBM_EQ = D3Q27_EQ<TRAITS>> struct D3Q27_SRT {};
+
+template <typename D> struct Traits { using dreal = D; };
+using TraitsDP = Traits<double>;
+using TraitsSP = Traits<float>;
+template <typename T, typename C> struct NSE { using TRAITS = T; using COLL = C; };
+
+struct Indexer { template <int d> long getOverlap() const { return d + 1; } };
+struct Idx3 { long a, b, c; long x() const { return a; } long y() const { return b; } long z() const { return c; } };
+struct Dist { bool a, b, c; bool x() const { return a; } bool y() const { return b; } bool z() const { return c; } };
+static double f0[10], f1[10], m[10];
+static short mp[10];
+static double px[4], py[4], pz[4];
+struct Base {
+    bool even_iter = false; Indexer indexer; long XYZ = 777; double lbmViscosity = 0.0125;
+    double* dfs[3] = {f0, f1, nullptr}; double* dmacro = m; short* dmap = mp;
+    long X() { return 5; } long Y() { return 6; } long Z() { return 7; }
+};
+struct ConstInflow : Base { double fx = 1, fy = 2, fz = 3; double inflow_vx = 0.1, inflow_vy = 0.2, inflow_vz = 0.3; };
+struct Profile : Base { double fx = 0, fy = 0, fz = 0; double* inflow_vx = px; double* inflow_vy = py; double* inflow_vz = pz; };
+
+template <typename N, typename DATA>
+int unused_launch(DATA& d) { return tnl_lbm_amd::step<N>(d, Idx3{0, 0, 0}, Idx3{5, 6, 7}, Dist{false, false, false}, true); }
+
+int main()
+{
+    using namespace tnl_lbm_amd;
+    auto c1 = config_for<NSE<TraitsDP, D3Q27_CUM<TraitsDP>>>();
+    auto c2 = config_for<NSE<TraitsSP, D3Q27_MRT<TraitsSP>>>();
+    auto c3 = config_for<NSE<TraitsDP, D3Q27_KBC_N4<TraitsDP>>>();
+    auto c4 = config_for<NSE<TraitsSP, D3Q27_KBC_C1<TraitsSP>>>(TNL_LBM_STORE_I16_DEV);
+    std::printf("cfg %d %d %d %d | %d %d %d | %d | %d %d | %u %u\n", c1.collision, c1.storage, c1.pattern, c1.macro_real,
+                c2.collision, c2.storage, c2.macro_real, c3.collision, c4.collision, c4.storage, c1.struct_size, c1.version);
+    ConstInflow a; Profile b;
+    auto s = data_for(a);
+    auto t = data_for(b);
+    std::printf("data %ld %ld %ld %ld %ld %ld %ld %d %g %g %g %g %d %d %d %g %g %g %d\n", (long) s.X, (long) s.Y, (long) s.Z,
+                (long) s.ox, (long) s.oy, (long) s.oz, (long) s.XYZ, s.even_iter, s.lbm_viscosity, s.fx, s.fy, s.fz,
+                s.df[0] == f0, s.df[1] == f1, s.map == mp, s.inflow_vx, s.inflow_vy, s.inflow_vz, s.inflow_profile[0] == nullptr);
+    std::printf("profile %d %d %d %g\n", t.inflow_profile[0] == px, t.inflow_profile[1] == py, t.inflow_profile[2] == pz, t.inflow_vx);
+    std::printf("flags %u %u\n", flags_for(Dist{true, false, true}, true), flags_for(Dist{false, true, false}, false));
+    if (false) unused_launch<NSE<TraitsDP, D3Q27_CUM<TraitsDP>>>(a);
+    return 0;
+}
+'''
+
+
+def build(tmp_path, defines=(), extra=""):
+    if not shutil.which("g++"):
+        pytest.skip("g++ missing")
+    src = tmp_path / "t.cpp"
+    src.write_text(SRC + extra)
+    exe = tmp_path / "t"
+    cmd = ["g++", "-std=c++17", "-O1", *[f"-D{d}" for d in defines], f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+           f"-L{LIBDIR}", "-ltnl_lbm_amd", f"-Wl,-rpath,{LIBDIR}", "-Wl,--allow-shlib-undefined"]
+    return subprocess.run(cmd, capture_output=True, text=True), exe
+
+
+@pytest.mark.parametrize("defines,cum,pattern", [((), 0, 1), (("USE_GEIER_CUM_2017",), 1, 1),
+                                                 (("USE_GEIER_CUM_2017", "USE_GEIER_CUM_ANTIALIAS"), 2, 1),
+                                                 (("AA_PATTERN",), 0, 0)])
+def test_header_maps_reference_types(tmp_path, defines, cum, pattern):
+    if not (LIBDIR / "libtnl_lbm_amd.so").exists():
+        pytest.skip("library not built")
+    r, exe = build(tmp_path, defines)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    cfg = out[0].split()
+    # CUM variant, F64, pattern, F64 macro | MRT_LES, F32, F32 macro | KBC_N4 | KBC_C1, I16_DEV
+    assert cfg[1:5] == [str(cum), "0", str(pattern), "0"]
+    assert cfg[6:9] == ["3", "1", "1"]
+    assert cfg[10] == "7" and cfg[12:14] == ["8", "5"]
+    assert cfg[16] == "2"  # ABI version
+    data = out[1].split()[1:]
+    assert data[:7] == ["5", "6", "7", "1", "2", "3", "777"]
+    assert data[7:12] == ["0", "0.0125", "1", "2", "3"]
+    assert data[12:15] == ["1", "1", "1"]
+    assert data[15:19] == ["0.1", "0.2", "0.3", "1"]
+    assert out[2].split()[1:] == ["1", "1", "1", "0"]
+    assert out[3].split()[1:] == [str(1 | 2 | 8), str(4)]
+
+
+def test_operator_without_counterpart_does_not_compile(tmp_path):
+    r, _ = build(tmp_path, extra="\nauto bad = tnl_lbm_amd::config_for<NSE<TraitsDP, D3Q27_SRT<TraitsDP>>>();\n")
+    assert r.returncode != 0
+    assert "no counterpart" in r.stderr
