@@ -697,38 +697,72 @@ __global__ void __launch_bounds__(256) k_general(const StepArgs A)
 // ===========================================================================
 // Coupled NSE + ADE: cudaLBMKernel<NSE,ADE> (kernels.h:114-196)
 // ===========================================================================
+// The D3Q7 inputs of one site: its code, the 7 pulled populations, the
+// diffusivity and the source.  Loaded before the D3Q27 update of the same
+// site, so that their latency overlaps it (the compiler cannot move them above
+// the D3Q27 stores by itself).
+template <int ST>
+struct AdeIn
+{
+	int16_t gi;
+	typename Storage<ST>::real f[7];
+	typename Storage<ST>::real D, S;
+};
+
 template <int ST, int PATTERN>
-LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename Storage<ST>::real vx,
-					  typename Storage<ST>::real vy, typename Storage<ST>::real vz, const Nb& NX, const Nb& NY, const Nb& NZ)
+LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb& NX, const Nb& NY, const Nb& NZ,
+					  AdeIn<ST>& in)
 {
 	using K = Ade7<ST>;
 	using real = typename K::real;
 	using elem = typename K::elem;
 	constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
 	const int16_t gi = B.map ? B.map[s0] : (int16_t) A_PERIODIC;
+	in.gi = gi;
 	const real pr = (real) B.phi_ref, sc = (real) B.scale;
 	auto macro_at = [&](const void* p) -> real {
 		return A.macro_f64 ? (real) ((const double*) p)[s0] : (real) ((const float*) p)[s0];
 	};
 	// D3Q7_MACRO_Default::copyQuantities (d3q7/macro.h:38-42)
-	const real D = B.diff ? macro_at(B.diff) : (real) B.D;
-	real phi = 0;
+	in.D = B.diff ? macro_at(B.diff) : (real) B.D;
+	in.S = B.source ? macro_at(B.source) : (real) 0;
 	if (gi != A_NOTHING) {
 		const elem* cur = (const elem*) B.df0;
 		auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
 		Nb PX = NX;
 		if (gi == A_OUTFLOW_RIGHT)
 			PX.p = PX.o = NX.m;
-		real f[7];
 #pragma unroll
 		for (int q = 0; q < 7; q++) {
 			if (AA && B.even)
-				f[q] = K::ld(cur + q * A.XYZ + s0, q, pr, sc);
+				in.f[q] = K::ld(cur + q * A.XYZ + s0, q, pr, sc);
 			else if (AA)
-				f[q] = K::ld(cur + opp(q) * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), pr, sc);
+				in.f[q] = K::ld(cur + opp(q) * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), pr, sc);
 			else
-				f[q] = K::ld(cur + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, pr, sc);
+				in.f[q] = K::ld(cur + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, pr, sc);
 		}
+	}
+}
+
+template <int ST, int PATTERN>
+LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename Storage<ST>::real vx,
+					  typename Storage<ST>::real vy, typename Storage<ST>::real vz, const Nb& NX, const Nb& NY, const Nb& NZ,
+					  const AdeIn<ST>& in)
+{
+	using K = Ade7<ST>;
+	using real = typename K::real;
+	using elem = typename K::elem;
+	constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
+	const int16_t gi = in.gi;
+	const real pr = (real) B.phi_ref, sc = (real) B.scale;
+	const real D = in.D;
+	real phi = 0;
+	if (gi != A_NOTHING) {
+		auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
+		real f[7];
+#pragma unroll
+		for (int q = 0; q < 7; q++)
+			f[q] = in.f[q];
 		switch (gi) {
 			case A_INFLOW:
 				phi = (real) B.inflow_phi;
@@ -754,10 +788,9 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 		if (gi == A_FLUID || gi == A_PERIODIC || gi == A_SOLID || gi == A_OUTFLOW_RIGHT) {
 			K::collide(B.collision, f, phi, vx, vy, vz, D);
 			if (B.source) {
-				const real S = macro_at(B.source);
 #pragma unroll
 				for (int q = 0; q < 7; q++)
-					f[q] += (real) w7(q) * S;
+					f[q] += (real) w7(q) * in.S;
 			}
 		}
 		elem* out = (elem*) (AA ? B.df0 : B.df1);
@@ -785,11 +818,19 @@ __global__ void __launch_bounds__(256) k_nse_ade(const StepArgs A, const AdeArgs
 	int64_t x, z, y;
 	if (! block_coords(A, x, z, y))
 		return;
+	constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
+	const int64_t s0 = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys + (y + A.oy);
+	// neighbours as General::site forms them (periodic wrap for PERIODIC-coded sites)
+	const bool per = (A.map ? A.map[s0] : (int16_t) GEO_PERIODIC) == GEO_PERIODIC;
+	const Nb NX = neighbours<AA>(x, A.X, A.ox, A.dist_x, per);
+	const Nb NY = neighbours<AA>(y, A.Y, A.oy, A.dist_y, per);
+	const Nb NZ = neighbours<AA>(z, A.Z, A.oz, A.dist_z, per);
+	AdeIn<ST> in;
+	ade_load<ST, PATTERN>(A, B, s0, NX, NY, NZ, in);
 	typename Storage<ST>::real vx, vy, vz;
-	Nb NX, NY, NZ;
-	General<ST, COLL, PATTERN>::site(A, x, y, z, vx, vy, vz, NX, NY, NZ);
-	const int64_t s0 = (NX.o * A.Zs + NZ.o) * A.Ys + NY.o;
-	ade_site<ST, PATTERN>(A, B, s0, vx, vy, vz, NX, NY, NZ);
+	Nb GX, GY, GZ;
+	General<ST, COLL, PATTERN>::site(A, x, y, z, vx, vy, vz, GX, GY, GZ);
+	ade_site<ST, PATTERN>(A, B, s0, vx, vy, vz, NX, NY, NZ, in);
 }
 
 template <int ST>
