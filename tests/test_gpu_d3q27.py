@@ -182,6 +182,68 @@ def test_deviation_storage_matches_oracle(storage, tol):
             assert rel_max(dg, dh) < tol, (storage, t)
 
 
+def split_step(lat, cuts, output_macro):
+    """One A-A step of the whole lattice as several launches over y ranges (the
+    A-A update is in place and each slot is read and written by one site, so
+    any partition gives the same result)."""
+    it = lat.iterations
+    for lo, hi in cuts:
+        lat.iterations = it
+        lat.step(output_macro=output_macro, offset=(0, lo, 0), end=(lat.X, hi, lat.Z))
+
+
+@pytest.mark.parametrize("storage", ["f16dev", "bf16dev", "i16dev"])
+@pytest.mark.parametrize("shape", [(6, 70, 5), (3, 260, 2), (4, 2, 3), (2, 130, 4)])
+def test_paired_kernel_bitwise(storage, shape):
+    """On the even A-A step the 2-byte forms update two sites per lane where
+    the y range allows it (even Y, offset and length).  Launches over y
+    ranges with odd bounds take the one-site kernel, so splitting the same
+    step at an odd y compares the two kernels; splitting at an even y checks
+    the paired kernel on partial rows.  Odd steps run the one-site kernel in
+    all three lattices.
+    The two kernels run the same FP32 collision, but the compiler may contract
+    a different multiply-add into an FMA in each, which moves a value by one
+    FP32 ulp and, at a near-tie, its 16-bit rounding by one step: allowed on
+    a few slots.  A pairing mistake (halves swapped, a wrong wrap) moves whole
+    populations (~1e-3) between sites."""
+    X, Y, Z = shape
+
+    def off_by(a, b):
+        """Slots further apart than one 16-bit step: for the float forms one
+        spacing at the larger magnitude, or binary16's subnormal spacing 2^-24
+        (bfloat16: 1e-9, for values that an FP32 ulp of the populations' scale
+        moves across zero)."""
+        if storage == "i16dev":
+            return (a.view(torch.int16).int() - b.view(torch.int16).int()).abs() > 1
+        a, b = a.double(), b.double()
+        step, floor = (2.0**-10, 2.0**-24) if storage == "f16dev" else (2.0**-7, 1e-9)
+        return (a - b).abs() > torch.maximum(a.abs(), b.abs()) * step + floor
+
+    lats = [D3Q27Lattice(X, Y, Z, storage=storage, nu=0.01, force=(1e-5, -2e-5, 3e-6), macro_real=_abi.REAL_F32,
+                         store_range=0.5) for _ in range(3)]
+    r = fields(X, Y, Z, 21, amp=0.03)
+    for g in lats:
+        g.set_equilibrium_field(*r)
+    odd = [(0, 1), (1, Y)] if Y == 2 else [(0, Y // 2 + (Y // 2) % 2 - 1), (Y // 2 + (Y // 2) % 2 - 1, Y)]
+    even = [(0, Y)] if Y == 2 else [(0, 2 * (Y // 4)), (2 * (Y // 4), Y)]
+    for t in range(6):
+        om = t % 2 == 1
+        lats[0].step(output_macro=om)
+        split_step(lats[1], odd, om)
+        split_step(lats[2], even, om)
+        torch.cuda.synchronize()
+        for k in (1, 2):
+            a, b = lats[0].df[0], lats[k].df[0]
+            assert not bool(off_by(a, b).any()), (t, k)
+            assert float((a.view(torch.int16) != b.view(torch.int16)).double().mean()) < 2e-3, (t, k)
+            if om:
+                m0, mk = lats[0].macro, lats[k].macro
+                assert float((m0 - mk).abs().max()) <= 1e-6 * float(m0.abs().max()), (t, k)
+        # continue both runs from the same state, so that differences do not accumulate
+        for k in (1, 2):
+            lats[k].df[0].copy_(lats[0].df[0])
+
+
 @pytest.mark.parametrize("pattern,geo_kind", [("AA", None), ("AB", "channel")])
 def test_i16_class_ranges_match_oracle(pattern, geo_kind):
     """The fixed-point form with one bound per direction class (rest, faces,

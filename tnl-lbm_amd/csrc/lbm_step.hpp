@@ -208,6 +208,11 @@ LBM_DEV char* after(char* p, const T& v)
 #ifndef LBM_WAVES_F64
 #define LBM_WAVES_F64 1
 #endif
+//  LBM_PAIR_EVEN bitmask over the 2-byte storage forms: two sites per lane
+//                on the even step (k_aa_even_pair)
+#ifndef LBM_PAIR_EVEN
+#define LBM_PAIR_EVEN 0x38
+#endif
 #ifndef LBM_MAXWAVES_F64
 #define LBM_MAXWAVES_F64 3
 #endif
@@ -328,6 +333,104 @@ k_aa_periodic(const StepArgs A)
 		}
 		if (A.out_macro)
 			write_macro(A, (xs[1] * A.Zs + zs[1]) * A.Ys + yl, rho, vx, vy, vz);
+	}
+}
+
+// ===========================================================================
+// Paired even step for the 2-byte storage forms (the A-A even step above).
+// Each lane updates the two sites y, y + 1 of a row (y even) and moves both
+// through one aligned 4-byte access per population, site y in the low half:
+// half the memory instructions per site.  The host uses it where Y, oy, offy,
+// ny and XYZ are even.  (The odd step keeps one site per lane: its +-1
+// shifted pairs would need 2-byte aligned 4-byte accesses, which measured
+// slower than the single 2-byte accesses they replace.)
+// ===========================================================================
+template <int ST, int COLL>
+struct PairCodec
+{
+	using S = Storage<ST>;
+	using elem = typename S::elem;
+	static_assert(sizeof(elem) == 2, "pairs of 2-byte elements");
+	static LBM_DEV float get(uint32_t h, int q, const StepArgs& A)
+	{
+		if constexpr (S::fixed)
+			return (float) (int16_t) (uint16_t) h * A.cls_sc[ncls(q)];
+		else
+			return (float) __builtin_bit_cast(elem, (uint16_t) h);
+	}
+	static LBM_DEV uint32_t put(float v, int q, const StepArgs& A)
+	{
+		if constexpr (S::fixed)
+			return (uint16_t) __builtin_amdgcn_cvt_pknorm_i16(v * A.cls_inv[ncls(q)], 0.0f)[0];
+		else
+			return __builtin_bit_cast(uint16_t, (elem) (v * out_factor<ST, COLL>(q)));
+	}
+};
+
+// Collides the pair held in w[] in place: site s (low halves), then s + 1.
+// Site s's outputs replace the low halves while the high halves still hold
+// site s + 1's inputs, so one packed array serves both sites.
+template <int ST, int COLL>
+LBM_DEV void collide_pair(uint32_t* w, const StepArgs& A, int64_t s)
+{
+	using C = PairCodec<ST, COLL>;
+	float f[27];
+	float rho, vx, vy, vz;
+#pragma unroll
+	for (int q = 0; q < 27; q++)
+		f[q] = C::get(w[q] & 0xffffu, q, A);
+	collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
+	if (A.out_macro)
+		write_macro(A, s, rho, vx, vy, vz);
+#pragma unroll
+	for (int q = 0; q < 27; q++)
+		w[q] = (w[q] & 0xffff0000u) | C::put(f[q], q, A);
+#pragma unroll
+	for (int q = 0; q < 27; q++)
+		f[q] = C::get(w[q] >> 16, q, A);
+	collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
+	if (A.out_macro)
+		write_macro(A, s + 1, rho, vx, vy, vz);
+#pragma unroll
+	for (int q = 0; q < 27; q++)
+		w[q] = (w[q] & 0xffffu) | (C::put(f[q], q, A) << 16);
+}
+
+template <int ST, int COLL>
+__global__ void __launch_bounds__(256) k_aa_even_pair(const StepArgs A)
+{
+	constexpr bool NT = (LBM_NT_EVEN >> ST) & 1;
+	constexpr bool PIN = (LBM_PIN_ADDR >> ST) & 1;
+	const uint32_t b = blockIdx.x;
+	const uint32_t L = (b % NXCD) * A.per_xcd + b / NXCD;
+	if (L >= A.nblocks)
+		return;
+	const uint32_t t = L / A.nyb;
+	const int64_t z = A.offz + (int64_t) (t % (uint32_t) A.nz);
+	const int64_t x = A.offx + (int64_t) (t / (uint32_t) A.nz);
+	const uint32_t pair = (L % A.nyb) * blockDim.x + threadIdx.x;
+	if (2 * pair >= (uint32_t) A.ny)
+		return;
+	const uint32_t yl = (uint32_t) (A.offy + 2 * (int64_t) pair + A.oy);  // sites yl, yl + 1 of the stored row
+	char* __restrict__ df = (char*) A.df0;
+	const uint64_t pb = A.plane_bytes;
+	const uint64_t row = (uint64_t) (((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys) * 2;
+	const uint32_t yb = yl * 2;
+	uint32_t w[27];
+#pragma unroll
+	for (int q = 0; q < 27; q++) {
+		const uint32_t* p = (const uint32_t*) (sptr<PIN>(df + ((uint64_t) q * pb + row)) + yb);
+		w[q] = NT ? __builtin_nontemporal_load(p) : *p;
+	}
+	collide_pair<ST, COLL>(w, A, (int64_t) (row / 2) + yl);
+	char* out = after<PIN>(df, w[26]);
+#pragma unroll
+	for (int q = 0; q < 27; q++) {
+		uint32_t* p = (uint32_t*) (sptr<PIN>(out + ((uint64_t) opp(q) * pb + row)) + yb);
+		if (NT)
+			__builtin_nontemporal_store(w[q], p);
+		else
+			*p = w[q];
 	}
 }
 
@@ -685,8 +788,11 @@ struct General
 	}
 };
 
+// The FP64 default cumulant needs ~170 VGPRs here: asking for 3 waves per SIMD
+// (<= 168) costs no spills; the alternative operators keep their registers.
 template <int ST, int COLL, int PATTERN, int TAPE = 0>
-__global__ void __launch_bounds__(256) k_general(const StepArgs A)
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(COLL == TNL_LBM_COLL_CUM ? 3 : 1, 8))) k_general(const StepArgs A)
 {
 	int64_t x, z, y;
 	if (! block_coords(A, x, z, y))
@@ -1048,10 +1154,35 @@ inline void linearize(StepArgs& A, unsigned& threads, unsigned& grid)
 	grid = padded;
 }
 
+// Launch shape of k_aa_even_pair (one lane per two sites of a row); false
+// where the pairs would not be 4-byte aligned.
+inline bool pair_shape(const StepArgs& A, StepArgs& B, unsigned& threads, unsigned& grid)
+{
+	if (((A.ny | A.offy | A.oy | A.Y | A.XYZ) & 1) || ((uintptr_t) A.df0 & 3))
+		return false;
+	B = A;
+	const uint32_t np = (uint32_t) A.ny / 2;
+	threads = np > 128 ? 256 : (np > 64 ? 128 : 64);
+	B.nyb = (np + threads - 1) / threads;
+	B.nblocks = B.nyb * (uint32_t) A.nz * (uint32_t) A.nx;  // no more blocks than one lane per site needs
+	const uint32_t padded = (B.nblocks + NXCD - 1) / NXCD * NXCD;
+	B.per_xcd = padded / NXCD;
+	grid = padded;
+	return true;
+}
+
 template <int ST, int COLL>
 int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
 {
 	if (! A.map && cfg->pattern == TNL_LBM_PATTERN_AA) {
+		if constexpr (sizeof(typename Storage<ST>::elem) == 2 && ((LBM_PAIR_EVEN >> ST) & 1)) {
+			StepArgs B;
+			unsigned t2 = 0, g2 = 0;
+			if (A.even && pair_shape(A, B, t2, g2)) {
+				hipLaunchKernelGGL((k_aa_even_pair<ST, COLL>), dim3(g2), dim3(t2), 0, st, B);
+				return hip_check(hipGetLastError(), "step launch");
+			}
+		}
 		if (A.even)
 			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(grid), dim3(threads), 0, st, A);
 		else
