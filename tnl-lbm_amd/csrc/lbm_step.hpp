@@ -143,8 +143,9 @@ LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real r
 // slot), push to r + c_q (same slot).
 // Addresses: base + plane[q] (kernel argument, scalar) + row (scalar) + y
 // (per lane), one vector add per access.  The even step streams with
-// nontemporal hints; the odd step's +-1 shifted rows share cache lines
-// between neighbouring waves and keep the default policy.
+// nontemporal hints; so do the odd step's accesses in the site's own y,
+// while its +-1 shifted rows share cache lines between neighbouring waves
+// and keep the default policy.
 // ===========================================================================
 template <int ST, int COLL>
 constexpr bool fast_collision = std::is_same<typename Storage<ST>::real, float>::value && COLL == TNL_LBM_COLL_CUM;
@@ -194,6 +195,11 @@ LBM_DEV char* after(char* p, const T& v)
 // Build-time tuning knobs (defaults are the measured best on MI355X):
 //  LBM_NT_EVEN   bitmask over storage forms: nontemporal hints on the even step
 //  LBM_PIN_ADDR  bitmask over storage forms: scalar base pointers, even step
+//  LBM_NT_ODD    bitmask over storage forms: nontemporal hints on the odd
+//                step's accesses within the site's own y (c_y = 0); the +-1
+//                shifted rows share lines with the neighbouring wave through
+//                L2 and keep the default policy (with hints they measured 40%
+//                slower)
 //  LBM_PIN_ODD   bitmask over storage forms: scalar base pointers, odd step
 //  LBM_WAVES_F64 minimum waves per SIMD requested for the FP64 kernels
 #ifndef LBM_NT_EVEN
@@ -202,8 +208,11 @@ LBM_DEV char* after(char* p, const T& v)
 #ifndef LBM_PIN_ADDR
 #define LBM_PIN_ADDR 0x3f
 #endif
+#ifndef LBM_NT_ODD
+#define LBM_NT_ODD 0x3f
+#endif
 #ifndef LBM_PIN_ODD
-#define LBM_PIN_ODD 0x00
+#define LBM_PIN_ODD 0x27
 #endif
 #ifndef LBM_WAVES_F64
 #define LBM_WAVES_F64 1
@@ -246,6 +255,7 @@ k_aa_periodic(const StepArgs A)
 	constexpr bool NT = (LBM_NT_EVEN >> ST) & 1;
 	constexpr bool PIN = (LBM_PIN_ADDR >> ST) & 1;
 	constexpr bool PIN_ODD = (LBM_PIN_ODD >> ST) & 1;
+	constexpr bool NT_ODD = (LBM_NT_ODD >> ST) & 1;
 	int64_t x, z, y;
 	if (! block_coords<false>(A, x, z, y))
 		return;
@@ -308,7 +318,13 @@ k_aa_periodic(const StepArgs A)
 #pragma unroll
 		for (int q = 0; q < 27; q++) {
 			const uint64_t row = (uint64_t) ((xs[1 - CX[q]] * A.Zs + zs[1 - CZ[q]]) * A.Ys) * E;
-			f[q] = S::ld((const elem*) (sptr<PIN_ODD>(df + ((uint64_t) opp(q) * pb + row)) + ys[1 - CY[q]]), A.cls_sc[ncls(q)]);
+			const elem* p = (const elem*) (sptr<PIN_ODD>(df + ((uint64_t) opp(q) * pb + row)) + ys[1 - CY[q]]);
+			if (NT_ODD && CY[q] == 0) {
+				const elem v = ld_nt(p);
+				f[q] = S::ld(&v, A.cls_sc[ncls(q)]);
+			}
+			else
+				f[q] = S::ld(p, A.cls_sc[ncls(q)]);
 		}
 		real rho, vx, vy, vz;
 		collide_site<ST, COLL>(f, A, rho, vx, vy, vz);
@@ -317,19 +333,30 @@ k_aa_periodic(const StepArgs A)
 			const uint64_t row = (uint64_t) ((xs[1 + CX[q]] * A.Zs + zs[1 + CZ[q]]) * A.Ys) * E;
 			return (elem*) (sptr<PIN_ODD>(out + ((uint64_t) q * pb + row)) + ys[1 + CY[q]]);
 		};
+		auto put = [&](int q, elem v) {
+			if (NT_ODD && CY[q] == 0)
+				st_nt(dst(q), v);
+			else
+				*dst(q) = v;
+		};
 		if constexpr (S::fixed) {
+			// two populations per conversion
 #pragma unroll
 			for (int q = 0; q < 27; q += 2) {
+				const int q1 = q + 1 < 27 ? q + 1 : q;
+				const auto r = __builtin_amdgcn_cvt_pknorm_i16(f[q] * A.cls_inv[ncls(q)], f[q1] * A.cls_inv[ncls(q1)]);
+				put(q, (elem) r[0]);
 				if (q + 1 < 27)
-					S::st2(dst(q), f[q], A.cls_inv[ncls(q)], dst(q + 1), f[q + 1], A.cls_inv[ncls(q + 1)]);
-				else
-					S::st(dst(q), f[q], A.cls_inv[ncls(q)]);
+					put(q1, (elem) r[1]);
 			}
 		}
 		else {
 #pragma unroll
-			for (int q = 0; q < 27; q++)
-				S::st(dst(q), f[q] * out_factor<ST, COLL>(q));
+			for (int q = 0; q < 27; q++) {
+				elem v;
+				S::st(&v, f[q] * out_factor<ST, COLL>(q));
+				put(q, v);
+			}
 		}
 		if (A.out_macro)
 			write_macro(A, (xs[1] * A.Zs + zs[1]) * A.Ys + yl, rho, vx, vy, vz);
