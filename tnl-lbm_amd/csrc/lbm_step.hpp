@@ -211,6 +211,10 @@ LBM_DEV char* after(char* p, const T& v)
 #ifndef LBM_NT_ODD
 #define LBM_NT_ODD 0x3f
 #endif
+//  LBM_NT_GENERAL bitmask over storage forms: the same hints in the map kernel
+#ifndef LBM_NT_GENERAL
+#define LBM_NT_GENERAL 0x3f
+#endif
 #ifndef LBM_PIN_ODD
 #define LBM_PIN_ODD 0x27
 #endif
@@ -538,6 +542,29 @@ struct General
 	static constexpr int EQK = eq_kind_of(COLL);
 	static constexpr bool FAST = fast_collision<ST, COLL>;
 
+	// Nontemporal hints on the streams no other wave touches in this step:
+	// the site's own slots, and the pulls and pushes with c_y = 0 (the +-1
+	// shifted rows share cache lines with the neighbouring wave)
+	static constexpr bool NT = (LBM_NT_GENERAL >> ST) & 1;
+	static LBM_DEV real ldq(const elem* p, float sc, bool nt)
+	{
+		if (nt) {
+			const elem v = ld_nt(p);
+			return S::ld(&v, sc);
+		}
+		return S::ld(p, sc);
+	}
+	static LBM_DEV void stq(elem* p, real v, float inv, bool nt)
+	{
+		if (nt) {
+			elem e;
+			S::st(&e, v, inv);
+			st_nt(p, e);
+		}
+		else
+			S::st(p, v, inv);
+	}
+
 	static LBM_DEV int64_t sidx(const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
 	{
 		return (xs * A.Zs + zs) * A.Ys + ys;
@@ -583,7 +610,7 @@ struct General
 							 Nb& NY, Nb& NZ)
 	{
 		const int64_t s0 = sidx(A, x + A.ox, y + A.oy, z + A.oz);
-		const int16_t gi = A.map ? A.map[s0] : (int16_t) GEO_PERIODIC;
+		const int16_t gi = A.map ? __builtin_nontemporal_load(A.map + s0) : (int16_t) GEO_PERIODIC;
 		const bool per = gi == GEO_PERIODIC;
 		NX = neighbours<AA>(x, A.X, A.ox, A.dist_x, per);
 		NY = neighbours<AA>(y, A.Y, A.oy, A.dist_y, per);
@@ -613,17 +640,18 @@ struct General
 			else if (AA && A.even) {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX.o, NY.o, NZ.o), fx_sc(q, A.rs));
+					f[q] = ldq(cur + q * A.XYZ + sidx(A, PX.o, NY.o, NZ.o), fx_sc(q, A.rs), NT);
 			}
 			else if (AA) {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + opp(q) * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(opp(q), A.rs));
+					f[q] = ldq(cur + opp(q) * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(opp(q), A.rs),
+							   NT && CY[q] == 0);
 			}
 			else {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs));
+					f[q] = ldq(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs), NT && CY[q] == 0);
 			}
 
 			// the raw moments of the pulled populations, recorded after the step's own stores
@@ -784,26 +812,26 @@ struct General
 				elem* __restrict__ out = (elem*) A.df0;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					S::st(out + opp(q) * A.XYZ + s0, f[q], fx_inv(opp(q), A.invr));
+					stq(out + opp(q) * A.XYZ + s0, f[q], fx_inv(opp(q), A.invr), NT);
 			}
 			else if (AA) {
 				elem* __restrict__ out = (elem*) A.df0;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					S::st(out + q * A.XYZ + sidx(A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q], fx_inv(q, A.invr));
+					stq(out + q * A.XYZ + sidx(A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q], fx_inv(q, A.invr), NT && CY[q] == 0);
 			}
 			else {
 				elem* __restrict__ out = (elem*) A.df1;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					S::st(out + q * A.XYZ + s0, f[q], fx_inv(q, A.invr));
+					stq(out + q * A.XYZ + s0, f[q], fx_inv(q, A.invr), NT);
 			}
 			if constexpr (TAPE != 0) {
 				if (rec) {
 					TT* __restrict__ tp = (TT*) A.tape;
 #pragma unroll
 					for (int k = 0; k < 10; k++)
-						tp[tape_index(k, s0)] = tm[k];
+						__builtin_nontemporal_store(tm[k], tp + tape_index(k, s0));
 				}
 			}
 		}
@@ -816,7 +844,8 @@ struct General
 };
 
 // The FP64 default cumulant needs ~170 VGPRs here: asking for 3 waves per SIMD
-// (<= 168) costs no spills; the alternative operators keep their registers.
+// (<= 168) costs at most 2 spilled registers; the alternative operators keep
+// their registers.
 template <int ST, int COLL, int PATTERN, int TAPE = 0>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(COLL == TNL_LBM_COLL_CUM ? 3 : 1, 8))) k_general(const StepArgs A)
