@@ -432,7 +432,8 @@ def main():
     st = args.storage
     tau = 0.5015 if st == "f64" else 0.505
     head = bench_case(N, st, args.steps, args.warmup, tau=tau, store_range=0.5 if st == "i16dev" else 1.0)
-    kernel = f"k_aa_periodic<{st.upper()},CUM> (even/odd)"
+    kernel = (f"k_aa_even_pair<{st.upper()},CUM> (even) / k_aa_periodic<{st.upper()},CUM> (odd)"
+              if st in ("f16dev", "bf16dev", "i16dev") else f"k_aa_periodic<{st.upper()},CUM> (even/odd)")
     workload = f"D3Q27 cumulant A-A {st.upper()}, {N}^3 periodic box, tau={tau}"
     out = {
         "metric": METRIC,
