@@ -58,6 +58,11 @@ def parse():
     return p.parse_args()
 
 
+def progress(msg):
+    """One line per finished case on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def timed_steps(lat, steps, warmup):
     """W untimed steps, then K steps bracketed by synchronisation; per-launch
     kernel durations from events recorded on the launch stream."""
@@ -458,6 +463,7 @@ def main():
                      "kernel_ms_odd": round(head["kernel_ms_odd"], 3)},
         "checks": {"mass_rel_drift": head["mass_rel_drift"], "nan_sites": head["nan_sites"]},
     }
+    progress(f"{workload}: {head['mlups']:.0f} MLUPS, {100 * head['roofline_frac']:.1f}% of 8 TB/s")
     if not args.no_precision:
         per = {}
         P = args.prec_size
@@ -468,11 +474,13 @@ def main():
                                  "kernel_ms_avg": round(r["kernel_ms_avg"], 3),
                                  "kernel_ms_even": round(r["kernel_ms_even"], 3),
                                  "kernel_ms_odd": round(r["kernel_ms_odd"], 3)}
+            progress(f"{pst} {P}^3: {r['mlups']:.0f} MLUPS, {100 * r['roofline_frac']:.1f}% of 8 TB/s")
         spec = []
         for ic in ("tgv", "turbulence"):
             errs, sp = precision_errors(args.err_size, args.err_steps, ("f32", "f32dev", "f16dev", "bf16dev", "i16dev"),
                                         ic=ic)
             spec.append(sp)
+            progress(f"precision errors ({ic}, {args.err_size}^3, {args.err_steps} steps) done")
             for pst, e in errs.items():
                 per[f"{pst}_{P}"].setdefault("error_vs_fp64", {})[ic] = {k: float(f"{v:.3e}") for k, v in e.items()}
         out["per_precision"] = per
@@ -483,8 +491,10 @@ def main():
                          ("f32", "hybrid"), ("f32", "segment")):
             adj[f"{st}_{mode}"] = adjoint_case(256, 96, 96, args.adjoint_steps, 200, storage=st, mode=mode,
                                                fd_dirs=args.adjoint_fd_dirs if (st, mode) == ("f64", "hybrid") else 0)
+            progress(f"adjoint {st} {mode}: ratio {adj[f'{st}_{mode}'].get('ratio')}")
         # coupled scalar: throughput at 384^3 in FP16 storage, accuracy against FP64 at 192^3
         adj["coupled_f16dev_384"] = coupled_adjoint_case(384, args.coupled_steps, "f16dev")
+        progress("coupled adjoint 384^3 done")
         r64, g64 = coupled_adjoint_case(192, args.coupled_steps, "f64", fd_dirs=args.adjoint_fd_dirs // 8,
                                         grad_out=True)
         r16, g16 = coupled_adjoint_case(192, args.coupled_steps, "f16dev", grad_out=True)
@@ -495,6 +505,7 @@ def main():
             "rel_L2": float(f"{float(torch.linalg.vector_norm(g16 - g64) / torch.linalg.vector_norm(g64)):.3e}"),
             "rel_max": float(f"{float((g16 - g64).abs().max() / g64.abs().max()):.3e}")}
         out["adjoint"] = adj
+        progress("coupled adjoint 192^3 done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
