@@ -20,6 +20,8 @@ are a closed-form function of the state the step read.
   feq(rho,u); after the collision j = j'/rho' of the interpolated set.
 - D3Q7 INFLOW and SYM_* (d3q7/bc.h): not collided; feq(phi_in, u) of the
   NSE velocity, and mirrored pulls.
+- D3Q7 OUTFLOW_RIGHT (d3q7/bc.h:73-92, 202): every population is pulled from
+  the column x - 1 and the site then collides like a fluid site.
 """
 import numpy as np
 import pytest
@@ -187,3 +189,39 @@ def test_d3q7_inflow_and_symmetry():
             want = p.copy()
             want[qidx((0, 0, -1))] = p[qidx((0, 0, 1))]
             assert np.array_equal(out[:, x, y, 6], want)
+
+
+def ade_raw(g):
+    """(7, X, Y, Z) populations -> the storage order of one D3Q7 array (y fastest)."""
+    return np.ascontiguousarray(g.transpose(0, 1, 3, 2)).reshape(-1)
+
+
+@pytest.mark.parametrize("coll", [_abi.ADE_COLL_SRT, _abi.ADE_COLL_MRT, _abi.ADE_COLL_CLBM])
+def test_d3q7_outflow_right_pulls_the_column_behind(coll):
+    """D3Q7 OUTFLOW_RIGHT moves the pull location of every population to x - 1
+    (xp = x = xm, d3q7/bc.h:73-74 and 90-92) and collides (bc.h:202).  A
+    periodic site pulls c_x = +1 from x - 1, c_x = 0 from x and c_x = -1 from
+    x + 1, so a periodic run whose columns x and x + 1 are replaced by the
+    column x - 1 must store the same bits at x, with the same NSE velocity
+    driving the collision."""
+    xo = 5
+    geo = np.full((X, Y, Z), GEO.PERIODIC, dtype=np.int16)
+    ageo = np.full((X, Y, Z), ADE_GEO.PERIODIC, dtype=np.int16)
+    ageo[xo] = ADE_GEO.OUTFLOW_RIGHT
+    rng = np.random.default_rng(9)
+    g0 = 0.1 + 0.05 * rng.random((7, X, Y, Z))
+    gb = g0.copy()
+    gb[:, xo] = g0[:, xo - 1]
+    gb[:, (xo + 1) % X] = g0[:, xo - 1]
+    outs = []
+    for amap, g in ((ageo, g0), (None, gb)):
+        h, _ = lattice(geo, seed=8)
+        a = O.HostAde(h, collision=coll, diffusivity=0.02, geo=amap)
+        a.df[0][:] = ade_raw(g)
+        h.step_coupled(a, output_macro=True)
+        outs.append(a.export_f64())
+    assert np.array_equal(outs[0][:, xo], outs[1][:, xo])
+    # the plane is not a copy of its input: it collided (the periodic run's
+    # uncollided pull of column x - 1 differs from the stored result)
+    pull = np.array([[[gb[q, xo - CX[q], (y - CY[q]) % Y, (z - CZ[q]) % Z] for z in range(Z)] for y in range(Y)] for q in range(7)])
+    assert np.abs(outs[0][:, xo] - pull).max() > 1e-6

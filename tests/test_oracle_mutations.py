@@ -28,6 +28,7 @@ MUTANTS = {
         "test_oracle_bc.py", "interpolating"),
     "d3q7_symmetry": ("case A_SYM_TOP: KS.f[zzm] = KS.f[zzp];", "case A_SYM_TOP: KS.f[zzp] = KS.f[zzm];",
                       "test_oracle_bc.py", "d3q7"),
+    "d3q7_outflow_pull": ("				xxp = xx = xm;", "				xxp = xm;", "test_oracle_bc.py", "outflow_right_pulls"),
     "relaxation_rate": ("const dreal omega1 = no1 / (no3 * KS.lbmViscosity + n1o2);",
                         "const dreal omega1 = no1 / (no3 * KS.lbmViscosity + no1);",
                         "test_oracle_lattice.py", "shear_wave"),
