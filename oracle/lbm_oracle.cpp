@@ -18,9 +18,9 @@
 // double, D3Q7 diffusion variance growth and advection, MRT_LES Smagorinsky
 // rate, the defining relations of the boundary codes INFLOW_LEFT,
 // INFLOW_BB_LEFT, OUTFLOW_RIGHT_INTERP, SYM_* and the D3Q7 INFLOW / SYM /
-// OUTFLOW_RIGHT codes, the adjoint by dot-product identities.  Partially
-// pinned (through invariants only): the 2017 cumulant limiters and the KBC
-// stabiliser.  The reference has no golden
+// OUTFLOW_RIGHT codes, the KBC stabiliser as the stationary point of the
+// quadratic entropy, the adjoint by dot-product identities.  Partially pinned
+// (through invariants only): the 2017 cumulant limiters.  The reference has no golden
 // vectors for this path (its only test fixtures are unrelated IBM matrices
 // stored as Git-LFS pointers).
 

@@ -11,7 +11,9 @@ retyped formula:
   tau_eff = (tau + sqrt(tau^2 + 18 C sqrt(Q) / rho)) / 2, tending to 1/tau;
 - every operator is equivariant under the 48 cube symmetries (index and
   sign slips in the 27 per-direction expressions break this);
-- a small shear wave decays as exp(-nu k^2 t) (KBC: nu = cs^2/2 (1/beta - 1)).
+- a small shear wave decays as exp(-nu k^2 t) (KBC: nu = cs^2/2 (1/beta - 1));
+- the KBC stabiliser gamma is the stationary point of the quadratic entropy
+  of the post-collision state (Karlin, Boesch, Chikatamarla 2014).
 """
 import numpy as np
 import pytest
@@ -121,3 +123,48 @@ def test_lattice_initialises_with_the_operators_equilibrium(name, kind):
     lat.set_equilibrium(1.02, 0.05, -0.04, 0.03)
     np.testing.assert_allclose(lat.export_f64()[:, 1, 2, 3], O.equilibrium(1.02, 0.05, -0.04, 0.03, kind=kind),
                                rtol=1e-15)
+
+
+def product_equilibrium(rho, u):
+    """Tensor product of the D1Q3 equilibria (moments 1, u, 1/3 + u^2 per
+    axis), the equilibrium of the KBC operators (col_kbc_n.h:289-330)."""
+    def phi(c, v):
+        return np.where(c == 0, 2.0 / 3.0 - v * v, (1.0 / 3.0 + v * v + c * v) / 2.0)
+    return rho * phi(CV[:, 0], u[0]) * phi(CV[:, 1], u[1]) * phi(CV[:, 2], u[2])
+
+
+def test_product_equilibrium_moments():
+    rho, u = 1.03, np.array([0.07, -0.04, 0.05])
+    feq = product_equilibrium(rho, u)
+    assert abs(feq.sum() - rho) < 1e-15
+    np.testing.assert_allclose(CV.T @ feq, rho * u, atol=1e-16)
+    np.testing.assert_allclose([(CV[:, d] ** 2 * feq).sum() for d in range(3)], rho * (1 / 3 + u * u), rtol=1e-14)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("name", list(KBC))
+def test_kbc_stabiliser_is_the_entropic_stationary_point(name, seed):
+    """KBC: f* = f - beta (2 ds + gamma dh) with f - feq = ds + dh, and gamma
+    chosen where the quadratic entropy sum (f* - feq)^2 / feq is stationary in
+    gamma, i.e. <dh | f* - feq> = 0 (weights 1/feq).  With a = f - feq and
+    b = f - f* = beta (2 a + (gamma - 2) dh), dh is parallel to
+    r = b / beta - 2 a for any split of a into ds and dh, so the output must
+    satisfy <r | f* - feq> = 0 with beta = 1 / (2 nu / cs^2 + 1).  This pins the
+    stabiliser and beta; r must not vanish (gamma = 2 is plain BGK, for which
+    the identity holds trivially)."""
+    rng = np.random.default_rng(seed)
+    nu = 0.01
+    f = near_eq(rng, 1.02, (0.05, -0.03, 0.04), 0.03)
+    rho = f.sum()
+    u = CV.T @ f / rho
+    g, _ = O.collide(f, nu, KBC[name])
+    feq = product_equilibrium(rho, u)
+    beta = 1.0 / (2.0 * nu * 3.0 + 1.0)
+    a, b = f - feq, f - g
+    r = b / beta - 2.0 * a
+
+    def ip(x, y):
+        return float((x * y / feq).sum())
+
+    assert np.sqrt(ip(r, r)) > 1e-2 * np.sqrt(ip(a, a))
+    assert abs(ip(r, g - feq)) <= 1e-10 * np.sqrt(ip(r, r) * ip(g - feq, g - feq))

@@ -1,6 +1,7 @@
 """The oracle's pins catch plausible mistakes: each case plants one mistake
 in a copy of oracle/lbm_oracle.cpp (a mirrored direction, a sign, a source
-index, swapped weights, a wrong relaxation rate or equilibrium weight),
+index, swapped weights, a wrong pull column, a wrong relaxation rate or
+stabiliser),
 builds it, and runs the pin that is meant to catch it against the mutant,
 which must fail.  CPU only; nothing of the product is involved."""
 import shutil
@@ -29,6 +30,11 @@ MUTANTS = {
     "d3q7_symmetry": ("case A_SYM_TOP: KS.f[zzm] = KS.f[zzp];", "case A_SYM_TOP: KS.f[zzp] = KS.f[zzm];",
                       "test_oracle_bc.py", "d3q7"),
     "d3q7_outflow_pull": ("				xxp = xx = xm;", "				xxp = xm;", "test_oracle_bc.py", "outflow_right_pulls"),
+    "kbc_stabiliser": ("const dreal gamma = no1 / beta - (no2 - no1 / beta) * num / den;",
+                       "const dreal gamma = no1 / beta - (no2 - no1 / beta) * den / num;",
+                       "test_oracle_alt.py", "stationary_point"),
+    "kbc_bgk_limit": ("const dreal gamma = no1 / beta - (no2 - no1 / beta) * num / den;",
+                      "const dreal gamma = no2 + 0 * num / den;", "test_oracle_alt.py", "stationary_point"),
     "relaxation_rate": ("const dreal omega1 = no1 / (no3 * KS.lbmViscosity + n1o2);",
                         "const dreal omega1 = no1 / (no3 * KS.lbmViscosity + no1);",
                         "test_oracle_lattice.py", "shear_wave"),
