@@ -19,8 +19,9 @@
 // rate, the defining relations of the boundary codes INFLOW_LEFT,
 // INFLOW_BB_LEFT, OUTFLOW_RIGHT_INTERP, SYM_* and the D3Q7 INFLOW / SYM /
 // OUTFLOW_RIGHT codes, the KBC stabiliser as the stationary point of the
-// quadratic entropy, the adjoint by dot-product identities.  Partially pinned
-// (through invariants only): the 2017 cumulant limiters.  The reference has no golden
+// quadratic entropy, the 2017 limiter's form and lambda, the adjoint by
+// dot-product identities.  Partially pinned (through invariants only): the
+// 2017 rate formulas omega3-5 and the coefficients A, B.  The reference has no golden
 // vectors for this path (its only test fixtures are unrelated IBM matrices
 // stored as Git-LFS pointers).
 

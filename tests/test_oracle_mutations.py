@@ -35,6 +35,9 @@ MUTANTS = {
                        "test_oracle_alt.py", "stationary_point"),
     "kbc_bgk_limit": ("const dreal gamma = no1 / beta - (no2 - no1 / beta) * num / den;",
                       "const dreal gamma = no2 + 0 * num / den;", "test_oracle_alt.py", "stationary_point"),
+    "cum2017_limiter": ("const dreal lambda3 = (dreal) (0.01), lambda4 = (dreal) (0.01), lambda5 = (dreal) (0.01);",
+                        "const dreal lambda3 = (dreal) (0.01), lambda4 = (dreal) (0.01), lambda5 = (dreal) (0.01) / rho;",
+                        "test_oracle_site.py", "limiter_form"),
     "relaxation_rate": ("const dreal omega1 = no1 / (no3 * KS.lbmViscosity + n1o2);",
                         "const dreal omega1 = no1 / (no3 * KS.lbmViscosity + no1);",
                         "test_oracle_lattice.py", "shear_wave"),

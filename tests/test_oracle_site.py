@@ -216,3 +216,52 @@ def test_i16_class_ranges_round_trip():
     lat.import_f64(f2)
     sat = lat.export_f64() - W[:, None, None, None]
     assert np.allclose(np.abs(sat), (W * R)[:, None, None, None], rtol=1e-6)
+
+
+def moment_dual(i, j, k):
+    """Populations whose raw moments sum_q h_q cx^a cy^b cz^c (a, b, c <= 2)
+    are 1 for (i, j, k) and 0 for the other 26 (the D3Q27 moment matrix is
+    invertible)."""
+    E = np.array([[CX[q] ** a * CY[q] ** b * CZ[q] ** c for q in range(27)]
+                  for a in range(3) for b in range(3) for c in range(3)], dtype=float)
+    e = np.zeros(27)
+    e[9 * i + 3 * j + k] = 1.0
+    return np.linalg.solve(E, e)
+
+
+# third-order combinations with their own limiter (col_cum.h:198-227)
+LIMITED = {"120+102": (((1, 2, 0), 0.5), ((1, 0, 2), 0.5)), "210-012": (((2, 1, 0), 0.5), ((0, 1, 2), -0.5)),
+           "111": (((1, 1, 1), 1.0),)}
+
+
+@pytest.mark.parametrize("coll", [1, 2])
+@pytest.mark.parametrize("combo", list(LIMITED))
+def test_2017_limiter_form_and_lambda(coll, combo):
+    """The 2017 variant relaxes each limited third-order cumulant combination
+    C with omega = omega_n + (1 - omega_n) |C| / (rho lambda + |C|) (Geier et
+    al. 2017), so C* / C = (1 - omega_n) rho lambda / (rho lambda + |C|): its
+    inverse is linear in |C| with intercept / (slope rho) = lambda = 0.01.
+    At u = 0 a perturbation carrying only this combination leaves the other
+    cumulants as they were, and order-3 cumulants are the raw moments.  rho
+    differs from 1 so that rho lambda and lambda are told apart."""
+    rho, nu = 1.05, 0.01
+    feq = O.equilibrium(rho, 0.0, 0.0, 0.0, kind=0)
+    h = sum(wt * moment_dual(*m) for m, wt in LIMITED[combo])
+    sign = {m: (1.0 if wt > 0 else -1.0) for m, wt in LIMITED[combo]}
+
+    def combo_of(f):
+        return sum(sign[m] * (CX ** m[0] * CY ** m[1] * CZ ** m[2] * f).sum() for m in sign)
+
+    amps = np.array([2e-4, 2e-3, 1e-2, 4e-2])
+    inv = []
+    for a in amps:
+        f = feq + a * h
+        assert abs(combo_of(f) - a) < 1e-15
+        g, _ = O.collide(f, nu, coll)
+        inv.append(a / combo_of(g))
+    inv = np.array(inv)
+    S, I = np.polyfit(amps, inv, 1)
+    assert np.abs(I + S * amps - inv).max() < 1e-9 * np.abs(inv).max()
+    assert abs(I / (S * rho) / 0.01 - 1) < 1e-8
+    omega_n = 1 - 1 / I
+    assert 0 < omega_n < 2
