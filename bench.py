@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--no-precision", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--no-shard", action="store_true")
+    p.add_argument("--shard-slab", type=int, default=96, help="x extent of the one-GPU slab of the headline box")
     p.add_argument("--adjoint-steps", type=int, default=2000)
     p.add_argument("--adjoint-fd-dirs", type=int, default=16)
     p.add_argument("--coupled-steps", type=int, default=1000)
@@ -113,6 +115,47 @@ def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015
     red = lat.reduce_macro()
     res["mass_rel_drift"] = abs(red[0] / sites - 1.0)
     res["nan_sites"] = red[2]
+    del lat
+    torch.cuda.empty_cache()
+    return res
+
+
+def shard_case(N, slab, steps, warmup, tau=0.5015):
+    """One GPU's share of the FP64 N^3 box split into x-slabs over N/slab
+    GPUs: slab x N x N sites plus a ghost plane per side, stepped with
+    DISTRIBUTED_X.  The slab is its own periodic neighbour, so the halo
+    exchange (tnl_lbm_d3q27_halo_x: PULL after the even step, PUSH after the
+    odd one) runs as device-local plane copies, standing in for the xGMI
+    transfers of a real split.  Times the step kernel and the exchange
+    separately with events on the launch stream."""
+    import torch
+    from pytnl_lbm import D3Q27Lattice, _abi
+
+    lat = D3Q27Lattice(slab, N, N, storage="f64", nu=(tau - 0.5) / 3.0, overlap=(1, 0, 0))
+    lat.set_equilibrium(1.0, 0.02, -0.01, 0.015)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for i in range(warmup + steps):
+        e = ev[i - warmup] if i >= warmup else None
+        if e:
+            e[0].record()
+        lat.step(flags=_abi.FLAG_DISTRIBUTED_X)
+        if e:
+            e[1].record()
+        lat.halo_x(lat, _abi.HALO_PULL if lat.iterations % 2 == 1 else _abi.HALO_PUSH)
+        if e:
+            e[2].record()
+    torch.cuda.synchronize()
+    k = sum(a.elapsed_time(b) for a, b, _ in ev) / steps * 1e-3
+    h = sum(b.elapsed_time(c) for _, b, c in ev) / steps * 1e-3
+    sites = slab * N * N
+    face = 9 * N * N * 8  # bytes per face per exchange
+    res = {"workload": f"x-slab {slab}x{N}x{N} + 1 ghost plane per side (one GPU's share of {N}^3 on {N // slab} GPUs), "
+                       "FP64, A-A, DISTRIBUTED_X",
+           "mlups": round(sites / k / 1e6, 1), "kernel_ms_avg": round(k * 1e3, 3),
+           "roofline_frac": round(432 * sites / k / 1e9 / PEAK_HBM_GBS, 4),
+           "halo_ms_avg": round(h * 1e3, 4), "halo_bytes_per_step": 2 * face,
+           "halo_note": "device-local copies of the 2 x 9 boundary planes; over one xGMI link per face "
+                        f"(~153 GB/s) a face would take ~{face / 153e9 * 1e3:.2f} ms, overlappable with the interior"}
     del lat
     torch.cuda.empty_cache()
     return res
@@ -464,6 +507,10 @@ def main():
         "checks": {"mass_rel_drift": head["mass_rel_drift"], "nan_sites": head["nan_sites"]},
     }
     progress(f"{workload}: {head['mlups']:.0f} MLUPS, {100 * head['roofline_frac']:.1f}% of 8 TB/s")
+    if not args.no_shard and st == "f64" and N % args.shard_slab == 0:
+        out["shard"] = shard_case(N, args.shard_slab, min(args.steps, 40), 2)
+        progress(f"shard {out['shard']['workload']}: {out['shard']['mlups']:.0f} MLUPS, "
+                 f"halo {out['shard']['halo_ms_avg']} ms per step")
     if not args.no_precision:
         per = {}
         P = args.prec_size

@@ -286,6 +286,29 @@ int tnl_lbm_d3q7_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* 
 int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
                                double out[6], void* stream);
 
+/* ---- halo exchange of an x-slab split ------------------------------------
+ * One lattice split into x-slabs, each stored with a ghost plane per side
+ * (ox >= 1) and stepped with TNL_LBM_FLAG_DISTRIBUTED_X.  `lower`'s +x face
+ * meets `upper`'s -x face (lower == upper: a slab that is its own periodic
+ * neighbour).  Every face moves the nine slots with c_x != 0 that cross it,
+ * one contiguous (z, y) plane each:
+ *   TNL_LBM_HALO_PULL  boundary planes -> the neighbour's ghost plane, for
+ *                      the next step's pulls: before an A-B step (on df[0],
+ *                      the array that step reads), or after an A-A even step
+ *                      (then the slots hold opposite directions);
+ *   TNL_LBM_HALO_PUSH  A-A only, after an odd step: the values that step
+ *                      wrote into the ghost planes -> the neighbour's
+ *                      boundary plane.
+ * Replaces the per-slot DistributedNDArraySynchronizer passes of
+ * lbm_block.hpp:434-470 (sync directions df_sync_directions, defs.h:352-385,
+ * reversed on even A-A steps and shifted by one site on odd ones).  The
+ * copies are asynchronous on `stream`; the two lattices may live on
+ * different GPUs that have peer access.  Both must have the same Y, Z, oy,
+ * oz and storage form.                                                      */
+enum tnl_lbm_halo_phase { TNL_LBM_HALO_PULL = 0, TNL_LBM_HALO_PUSH = 1 };
+int tnl_lbm_d3q27_halo_x(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* lower,
+                         const tnl_lbm_d3q27_data* upper, int phase, void* stream);
+
 /* ---- adjoint of one step ------------------------------------------------
  * The transposed Jacobian of one tnl_lbm_d3q27_step with the default
  * cumulant and the A-B pattern, taken at the state SD->df[0] (the state

@@ -64,3 +64,24 @@ def test_validation_errors_without_gpu():
     cfg2 = _abi.Config.make(collision=_abi.COLL_CUM_2017, storage=_abi.STORE_F16_DEV)
     d.lbm_viscosity = 0.01
     assert lib.tnl_lbm_d3q27_step(C.byref(cfg2), C.byref(d), off, end, 0, None) == _abi.ERR_UNSUPPORTED
+
+
+def test_halo_validation_without_gpu():
+    """tnl_lbm_d3q27_halo_x rejects slabs without ghost planes, mismatched
+    faces, unknown phases and the push phase of A-B before any copy."""
+    lib = _abi.load_library()
+    cfg = _abi.Config.make()
+    lo = _abi.D3Q27Data.make(4, 6, 5, 1, 0, 0)
+    up = _abi.D3Q27Data.make(3, 6, 5, 1, 0, 0)
+    lo.df[0] = up.df[0] = 0x1000
+    flat = _abi.D3Q27Data.make(4, 6, 5)
+    flat.df[0] = 0x1000
+    assert lib.tnl_lbm_d3q27_halo_x(C.byref(cfg), C.byref(lo), C.byref(flat), _abi.HALO_PULL, None) == _abi.ERR_DIMS
+    other = _abi.D3Q27Data.make(4, 7, 5, 1, 0, 0)
+    other.df[0] = 0x1000
+    assert lib.tnl_lbm_d3q27_halo_x(C.byref(cfg), C.byref(lo), C.byref(other), _abi.HALO_PULL, None) == _abi.ERR_DIMS
+    assert lib.tnl_lbm_d3q27_halo_x(C.byref(cfg), C.byref(lo), C.byref(up), 7, None) == _abi.ERR_ARG
+    ab = _abi.Config.make(pattern=_abi.PATTERN_AB)
+    lo.df[1] = up.df[1] = 0x2000
+    assert lib.tnl_lbm_d3q27_halo_x(C.byref(ab), C.byref(lo), C.byref(up), _abi.HALO_PUSH, None) == _abi.ERR_UNSUPPORTED
+    assert lib.tnl_lbm_d3q27_halo_x(None, C.byref(lo), C.byref(up), _abi.HALO_PULL, None) == _abi.ERR_NULL

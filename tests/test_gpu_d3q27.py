@@ -244,6 +244,44 @@ def test_paired_kernel_bitwise(storage, shape):
             lats[k].df[0].copy_(lats[0].df[0])
 
 
+@pytest.mark.parametrize("nslabs", [1, 2])
+@pytest.mark.parametrize("pattern", ["AA", "AB"])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16dev"])
+def test_x_slabs_with_halo_match_the_periodic_box(storage, pattern, nslabs):
+    """One periodic box stepped whole, and the same box split into x-slabs
+    with a ghost plane per side, stepped with DISTRIBUTED_X and exchanged
+    through tnl_lbm_d3q27_halo_x (one slab: its own periodic neighbour; two:
+    a ring).  A-B pulls after a PULL exchange; A-A exchanges PULL after the
+    even step and PUSH after the odd one.  Every slab must hold the whole
+    box's bits after every step: the kernels read the same values from the
+    ghost planes as from the wrapped neighbours, and every pushed value
+    reaches its owner."""
+    X, Y, Z = 8, 70, 6
+    kw = dict(storage=storage, pattern=pattern, nu=0.01, force=(1e-5, -2e-5, 3e-6), macro_real=_abi.REAL_F32)
+    whole = D3Q27Lattice(X, Y, Z, **kw)
+    whole.set_equilibrium_field(*fields(X, Y, Z, 31, amp=0.03))
+    w = X // nslabs
+    slabs = [D3Q27Lattice(w, Y, Z, overlap=(1, 0, 0), **kw) for _ in range(nslabs)]
+    for k, s in enumerate(slabs):
+        s.stored(0)[:, 1:w + 1] = whole.stored(0)[:, k * w:(k + 1) * w]
+    ring = [(slabs[k], slabs[(k + 1) % nslabs]) for k in range(nslabs)]
+    for t in range(6):
+        if pattern == "AB":
+            for lo, up in ring:
+                lo.halo_x(up, _abi.HALO_PULL)
+        for s in slabs:
+            s.step(flags=_abi.FLAG_DISTRIBUTED_X)
+        if pattern == "AA":
+            for lo, up in ring:
+                lo.halo_x(up, _abi.HALO_PULL if t % 2 == 0 else _abi.HALO_PUSH)
+        whole.step()
+        torch.cuda.synchronize()
+        ref = whole.stored(whole.current_df())
+        for k, s in enumerate(slabs):
+            got = s.stored(s.current_df())[:, 1:w + 1].contiguous()
+            assert torch.equal(got.view(torch.uint8), ref[:, k * w:(k + 1) * w].contiguous().view(torch.uint8)), (t, k)
+
+
 @pytest.mark.parametrize("pattern,geo_kind", [("AA", None), ("AB", "channel")])
 def test_i16_class_ranges_match_oracle(pattern, geo_kind):
     """The fixed-point form with one bound per direction class (rest, faces,

@@ -454,6 +454,55 @@ int tnl_lbm_d3q27_import_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data
 	return transfer(cfg, SD, which, const_cast<double*>(in), false, stream);
 }
 
+int tnl_lbm_d3q27_halo_x(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* lower, const tnl_lbm_d3q27_data* upper,
+						 int phase, void* stream)
+{
+	if (int rc = validate(cfg, lower))
+		return rc;
+	if (int rc = validate(cfg, upper))
+		return rc;
+	if (lower->ox < 1 || upper->ox < 1)
+		return fail(TNL_LBM_ERR_DIMS, "an x-slab halo needs a ghost plane on each side (ox >= 1)");
+	if (lower->Y != upper->Y || lower->Z != upper->Z || lower->oy != upper->oy || lower->oz != upper->oz)
+		return fail(TNL_LBM_ERR_DIMS, "neighbouring slabs must have the same Y, Z, oy and oz");
+	if (phase != TNL_LBM_HALO_PULL && phase != TNL_LBM_HALO_PUSH)
+		return fail(TNL_LBM_ERR_ARG, "unknown halo phase");
+	const bool aa = cfg->pattern == TNL_LBM_PATTERN_AA;
+	if (phase == TNL_LBM_HALO_PUSH && ! aa)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the push phase belongs to the A-A odd step");
+	const uint64_t eb = (uint64_t) ELEM_BYTES[cfg->storage];
+	const int64_t plane = (lower->Y + 2 * lower->oy) * (lower->Z + 2 * lower->oz);
+	auto at = [&](const tnl_lbm_d3q27_data* D, int q, int64_t xs) {
+		return (char*) D->df[0] + ((uint64_t) q * (uint64_t) D->XYZ + (uint64_t) (xs * plane)) * eb;
+	};
+	// storage x of the planes next to the face
+	const int64_t lo_last = lower->ox + lower->X - 1, lo_ghost = lower->ox + lower->X;
+	const int64_t up_first = upper->ox, up_ghost = upper->ox - 1;
+	hipStream_t st = (hipStream_t) stream;
+	for (int q = 0; q < 27; q++) {
+		if (CX[q] == 0)
+			continue;
+		// Lower's ghost plane (its +x side) takes from upper's first plane the
+		// slots with c_x = +1 for A-A (the odd step reads slot opp(q) of the
+		// populations q that move in -x) and with c_x = -1 for A-B (pulled as
+		// they are); the other nine slots go from lower's last plane to
+		// upper's ghost plane.  The push phase runs the same pairs backwards.
+		const bool up_to_lo = aa ? CX[q] > 0 : CX[q] < 0;
+		char *dst, *src;
+		if (phase == TNL_LBM_HALO_PULL) {
+			dst = up_to_lo ? at(lower, q, lo_ghost) : at(upper, q, up_ghost);
+			src = up_to_lo ? at(upper, q, up_first) : at(lower, q, lo_last);
+		}
+		else {
+			dst = up_to_lo ? at(upper, q, up_first) : at(lower, q, lo_last);
+			src = up_to_lo ? at(lower, q, lo_ghost) : at(upper, q, up_ghost);
+		}
+		if (int rc = hip_check(hipMemcpyAsync(dst, src, (size_t) plane * eb, hipMemcpyDeviceToDevice, st), "halo copy"))
+			return rc;
+	}
+	return TNL_LBM_OK;
+}
+
 int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, double out[6], void* stream)
 {
 	if (int rc = validate(cfg, SD))

@@ -65,6 +65,10 @@ FLAG_DISTRIBUTED_X = 0x2
 FLAG_DISTRIBUTED_Y = 0x4
 FLAG_DISTRIBUTED_Z = 0x8
 
+# tnl_lbm_halo_phase
+HALO_PULL = 0
+HALO_PUSH = 1
+
 
 class GEO:
     """D3Q27_BC_All::GEO (d3q27/bc.h:17-43)."""
@@ -184,6 +188,7 @@ def _declare(lib, prefix: str):
         "d3q27_step_record": [cfgp, d27p, _P, C.c_int],
         "d3q27_adjoint_step_recorded": [cfgp, d27p, _P, C.c_int, _P, _P, _P],
         "d3q7_adjoint_step": [cfgp, d27p, d7p, _P, _P, _P, _P],
+        "d3q27_halo_x": [cfgp, d27p, d27p, C.c_int],
     }
     for name, args in sig.items():
         fn = getattr(lib, prefix + name, None)

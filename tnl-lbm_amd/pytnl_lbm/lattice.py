@@ -59,7 +59,7 @@ class D3Q27Lattice:
 
     def __init__(self, X: int, Y: int, Z: int, *, collision="CUM", storage="f64", pattern="AA",
                  macro_real=None, nu: float = 0.0, force=(0.0, 0.0, 0.0), geo=None, device="cuda:0",
-                 XYZ: int | None = None, store_range: float = 1.0, store_range_class=None):
+                 XYZ: int | None = None, store_range: float = 1.0, store_range_class=None, overlap=(0, 0, 0)):
         if not torch.cuda.is_available():
             raise RuntimeError("D3Q27Lattice needs a GPU (libtnl_lbm_amd.so runs on gfx950 only)")
         self.lib = _abi.load_library()
@@ -69,7 +69,8 @@ class D3Q27Lattice:
         if macro_real is None:
             macro_real = _abi.REAL_F64 if st == _abi.STORE_F64 else _abi.REAL_F32
         self.cfg = _abi.Config.make(coll, st, pat, macro_real)
-        self.d = _abi.D3Q27Data.make(X, Y, Z, XYZ=XYZ)
+        self.d = _abi.D3Q27Data.make(X, Y, Z, *overlap, XYZ=XYZ)
+        self.ox, self.oy, self.oz = overlap
         self.d.store_range = store_range
         if store_range_class is not None:
             for n in range(4):
@@ -182,6 +183,23 @@ class D3Q27Lattice:
 
     def current_df(self):
         return self.iterations % 2 if self.cfg.pattern == _abi.PATTERN_AB else 0
+
+    def halo_x(self, upper: "D3Q27Lattice", phase: int, stream=None):
+        """Halo exchange of an x-slab split (tnl_lbm_d3q27_halo_x): this slab's
+        +x face meets `upper`'s -x face (`upper` is this slab for a slab that
+        is its own periodic neighbour).  phase _abi.HALO_PULL before an A-B
+        step or after an A-A even step, _abi.HALO_PUSH after an A-A odd step.
+        Both slabs need overlap (1, .., ..) or more and step with
+        _abi.FLAG_DISTRIBUTED_X."""
+        for lat in (self, upper):
+            lat.update_kernel_data()  # df[0]: the array the next step reads
+        self._check(self.lib.tnl_lbm_d3q27_halo_x(C.byref(self.cfg), C.byref(self.d), C.byref(upper.d), phase,
+                                                  _stream_ptr(stream)))
+
+    def stored(self, which: int = 0) -> torch.Tensor:
+        """Population array `which` as a (27, X + 2ox, Z + 2oz, Y + 2oy) view of
+        the stored layout (q slowest, y fastest), overlaps included."""
+        return self.df[which].view(27, self.X + 2 * self.ox, self.Z + 2 * self.oz, self.Y + 2 * self.oy)
 
     # ---- adjoint ----------------------------------------------------------------
     def adjoint_step(self, lam_next: torch.Tensor, lam: torch.Tensor, grad_inflow: torch.Tensor | None = None,
