@@ -246,10 +246,10 @@ def test_paired_kernel_bitwise(storage, shape):
 
 @pytest.mark.parametrize("storage", ["f32", "f32dev", "f16dev", "i16dev"])
 def test_odd_step_block_size_bitwise(storage):
-    """The odd A-A step of the FP32-compute forms runs 512-lane blocks where
-    the row length is a multiple of 512; a launch over half the row takes the
-    256-lane blocks.  Both run the same per-site code, so stepping the lattice
-    whole and as two half-row launches must give the same bits."""
+    """The periodic A-A kernel of the FP32-compute forms runs 512-lane blocks
+    where the row length is a multiple of 512; a launch over half the row
+    takes the 256-lane blocks.  Both run the same per-site code, so stepping
+    the lattice whole and as two half-row launches must give the same bits."""
     X, Y, Z = 3, 512, 2
     kw = dict(storage=storage, nu=0.01, force=(1e-5, -2e-5, 3e-6), macro_real=_abi.REAL_F32, store_range=0.5)
     a, b = D3Q27Lattice(X, Y, Z, **kw), D3Q27Lattice(X, Y, Z, **kw)

@@ -218,12 +218,14 @@ LBM_DEV char* after(char* p, const T& v)
 #ifndef LBM_PIN_ODD
 #define LBM_PIN_ODD 0x27
 #endif
-//  LBM_ODD_TPB   block size of the odd step for the FP32-compute forms where
-//                the row length is a multiple of it: a 512-site row in one
-//                block keeps 7 of its 8 wave boundaries, whose +-1 shifted
-//                lines the neighbouring waves share, on one CU
-#ifndef LBM_ODD_TPB
-#define LBM_ODD_TPB 512
+//  LBM_TPB_FP32  block size of the periodic kernel for the FP32-compute forms
+//                where the row length is a multiple of it (the paired 2-byte
+//                even step keeps its own shape): a 512-site row in one block
+//                keeps 7 of its 8 wave boundaries, whose +-1 shifted lines the
+//                neighbouring waves share on the odd step, on one CU
+//                (measured +0.6 to +1.2 points on both steps at 512^3)
+#ifndef LBM_TPB_FP32
+#define LBM_TPB_FP32 512
 #endif
 #ifndef LBM_WAVES_F64
 #define LBM_WAVES_F64 1
@@ -255,7 +257,7 @@ LBM_DEV char* sptr(char* p)
 }
 
 template <int ST, int COLL, bool EVEN>
-__global__ void __launch_bounds__(ST == TNL_LBM_STORE_F64 ? 256 : LBM_ODD_TPB)
+__global__ void __launch_bounds__(ST == TNL_LBM_STORE_F64 ? 256 : LBM_TPB_FP32)
 __attribute__((amdgpu_waves_per_eu(ST == TNL_LBM_STORE_F64 ? LBM_WAVES_F64 : 1, ST == TNL_LBM_STORE_F64 ? LBM_MAXWAVES_F64 : 8)))
 k_aa_periodic(const StepArgs A)
 {
@@ -1246,16 +1248,19 @@ int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, 
 				return hip_check(hipGetLastError(), "step launch");
 			}
 		}
-		if (A.even)
-			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(grid), dim3(threads), 0, st, A);
-		else if (ST != TNL_LBM_STORE_F64 && LBM_ODD_TPB > 256 && A.ny % LBM_ODD_TPB == 0) {
+		if (ST != TNL_LBM_STORE_F64 && LBM_TPB_FP32 > 256 && A.ny % LBM_TPB_FP32 == 0) {
 			StepArgs B = A;
-			B.nyb = (uint32_t) A.ny / LBM_ODD_TPB;
+			B.nyb = (uint32_t) A.ny / LBM_TPB_FP32;
 			B.nblocks = B.nyb * (uint32_t) A.nz * (uint32_t) A.nx;  // fewer blocks than at 256
 			const uint32_t padded = (B.nblocks + NXCD - 1) / NXCD * NXCD;
 			B.per_xcd = padded / NXCD;
-			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(padded), dim3(LBM_ODD_TPB), 0, st, B);
+			if (A.even)
+				hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(padded), dim3(LBM_TPB_FP32), 0, st, B);
+			else
+				hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(padded), dim3(LBM_TPB_FP32), 0, st, B);
 		}
+		else if (A.even)
+			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(grid), dim3(threads), 0, st, A);
 		else
 			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(grid), dim3(threads), 0, st, A);
 	}
