@@ -463,10 +463,12 @@ __global__ void __launch_bounds__(256) k_aa_even_pair(const StepArgs A)
 		w[q] = NT ? __builtin_nontemporal_load(p) : *p;
 	}
 	collide_pair<ST, COLL>(w, A, (int64_t) (row / 2) + yl);
-	char* out = after<PIN>(df, w[26]);
 #pragma unroll
 	for (int q = 0; q < 27; q++) {
-		uint32_t* p = (uint32_t*) (sptr<PIN>(out + ((uint64_t) opp(q) * pb + row)) + yb);
+		// scalar base pinned after w[q] is final, 32-bit lane offset (yl is
+		// even: pair yl / 2 of the row): the global saddr store form, where
+		// one base after the last value left 64-bit vector adds per store
+		uint32_t* p = at((uint32_t*) sptr<PIN>(after<PIN>(df, w[q]) + ((uint64_t) opp(q) * pb + row)), yl / 2);
 		if (NT)
 			__builtin_nontemporal_store(w[q], p);
 		else
