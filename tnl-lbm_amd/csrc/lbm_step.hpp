@@ -463,12 +463,16 @@ __global__ void __launch_bounds__(256) k_aa_even_pair(const StepArgs A)
 		w[q] = NT ? __builtin_nontemporal_load(p) : *p;
 	}
 	collide_pair<ST, COLL>(w, A, (int64_t) (row / 2) + yl);
+	// The stores use the loads' lane offset, kept opaque: when the compiler
+	// sees the two equal it shares 64-bit lane pointers between loads and
+	// stores and drops both out of the scalar-base (saddr) form.  Each store's
+	// base is pinned after its value is final (one base after the last value
+	// left 64-bit vector adds on every store).
+	uint32_t ob = yb;
+	asm volatile("" : "+v"(ob));
 #pragma unroll
 	for (int q = 0; q < 27; q++) {
-		// scalar base pinned after w[q] is final, 32-bit lane offset (yl is
-		// even: pair yl / 2 of the row): the global saddr store form, where
-		// one base after the last value left 64-bit vector adds per store
-		uint32_t* p = at((uint32_t*) sptr<PIN>(after<PIN>(df, w[q]) + ((uint64_t) opp(q) * pb + row)), yl / 2);
+		uint32_t* p = (uint32_t*) ((char*) sptr<PIN>(after<PIN>(df, w[q]) + ((uint64_t) opp(q) * pb + row)) + ob);
 		if (NT)
 			__builtin_nontemporal_store(w[q], p);
 		else
