@@ -113,6 +113,33 @@ def test_channel_bcs_match_oracle(pattern, inflow, outflow):
             assert rel_max(a.cpu().numpy(), b) < 5e-13  # small transverse velocities: FMA rounding
 
 
+@pytest.mark.parametrize("pattern,inflow,outflow", [("AA", GEO.INFLOW, GEO.OUTFLOW_EQ),
+                                                     ("AB", GEO.INFLOW_EQ_LEFT, GEO.OUTFLOW_RIGHT)])
+@pytest.mark.parametrize("storage,tol", [("f32dev", 2e-5), ("f16dev", 3e-3), ("bf16dev", 2e-2)])
+def test_deviation_storage_channel_bcs_match_oracle(storage, tol, pattern, inflow, outflow):
+    """The deviation storage forms through the map kernel: walls, inflow,
+    outflow and NOTHING sites next to fluid sites (the FP32 raw-moment
+    collision at fluid sites, the reference chain at the boundary codes),
+    against the oracle's FP32 deviation chain.  The bound for F32_DEV is
+    four times the periodic box's: INFLOW_EQ_LEFT extrapolates
+    rho = (1 + S_0 + 2 S_-x) / (1 - u_x) from O(1) FP32 quantities, whose
+    rounding (contracted on the GPU, not in the oracle) is ~1e-7 absolute,
+    up to ~7e-6 of this nearly uniform flow's deviation scale."""
+    X, Y, Z = 12, 9, 8
+    geo = channel_geo(X, Y, Z, inflow, outflow)
+    g, h = pair(X, Y, Z, 0, storage=storage, pattern=pattern, geo=geo, nu=0.02)
+    prof = np.linspace(0.01, 0.03, Y * Z).reshape(Z, Y)
+    for lat in (g, h):
+        lat.set_equilibrium(1.0, 0.02, 0.0, 0.0)
+        lat.set_inflow(0.02, 0.001, 0.0, profile=(prof, None, None))
+    Wb = W[:, None, None, None]
+    for t in range(6):
+        g.step(output_macro=True)
+        h.step(output_macro=True)
+        dg, dh = g.export_f64().cpu().numpy() - Wb, h.export_f64() - Wb
+        assert rel_max(dg, dh) < tol, (storage, pattern, t, rel_max(dg, dh))
+
+
 @pytest.mark.parametrize("sym", [GEO.SYM_TOP, GEO.SYM_BOTTOM, GEO.SYM_LEFT, GEO.SYM_RIGHT, GEO.SYM_BACK, GEO.SYM_FRONT])
 def test_symmetry_planes_match_oracle(sym):
     X, Y, Z = 6, 6, 6
