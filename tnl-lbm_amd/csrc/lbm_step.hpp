@@ -405,6 +405,26 @@ struct PairCodec
 		else
 			return __builtin_bit_cast(uint16_t, (elem) (v * out_factor<ST, COLL>(q)));
 	}
+	// forms whose conversion packs two values at once
+	static constexpr bool PACK2 = S::fixed || std::is_same<elem, __bf16>::value;
+	// the value the conversion takes, and the packed pair (lo = site y)
+	static LBM_DEV float scaled(float v, int q, const StepArgs& A)
+	{
+		if constexpr (S::fixed)
+			return v * A.cls_inv[ncls(q)];
+		else
+			return v * out_factor<ST, COLL>(q);
+	}
+	static LBM_DEV uint32_t pack2(float lo, float hi)
+	{
+		if constexpr (S::fixed)
+			return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pknorm_i16(lo, hi));
+		else {
+			typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+			const bf16x2 v = {(__bf16) lo, (__bf16) hi};
+			return __builtin_bit_cast(uint32_t, v);
+		}
+	}
 };
 
 // Collides the pair held in w[] in place: site s (low halves), then s + 1.
@@ -418,10 +438,11 @@ LBM_DEV void collide_pair(uint32_t* w, const StepArgs& A, int64_t s)
 	using C = PairCodec<ST, COLL>;
 	float f[27];
 	float rho, vx, vy, vz;
-	if constexpr (Storage<ST>::fixed) {
-		// Fixed point: site s's scaled outputs wait as floats, in the
-		// registers its decoded inputs held, and one conversion packs both
-		// sites: no per-half merges (1001 -> 947 VALU per pair).
+	if constexpr (C::PACK2) {
+		// I16 and BF16 convert two values per instruction: site s's scaled
+		// outputs wait as floats, in the registers its decoded inputs held,
+		// and one conversion packs both sites, with no per-half merges
+		// (I16: 1001 -> 947 VALU per pair).
 		float a[27];
 #pragma unroll
 		for (int q = 0; q < 27; q++)
@@ -431,7 +452,7 @@ LBM_DEV void collide_pair(uint32_t* w, const StepArgs& A, int64_t s)
 			write_macro(A, s, rho, vx, vy, vz);
 #pragma unroll
 		for (int q = 0; q < 27; q++)
-			a[q] = f[q] * A.cls_inv[ncls(q)];
+			a[q] = C::scaled(f[q], q, A);
 #pragma unroll
 		for (int q = 0; q < 27; q++)
 			f[q] = C::get(w[q] >> 16, q, A);
@@ -440,7 +461,7 @@ LBM_DEV void collide_pair(uint32_t* w, const StepArgs& A, int64_t s)
 			write_macro(A, s + 1, rho, vx, vy, vz);
 #pragma unroll
 		for (int q = 0; q < 27; q++)
-			w[q] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pknorm_i16(a[q], f[q] * A.cls_inv[ncls(q)]));
+			w[q] = C::pack2(a[q], C::scaled(f[q], q, A));
 		return;
 	}
 #pragma unroll
