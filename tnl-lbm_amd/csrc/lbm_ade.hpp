@@ -43,12 +43,13 @@ struct Ade7
 	using elem = typename S::elem;
 
 	// deviation forms store (f_q - w_q phi_ref) * scale; the fixed-point form
-	// stores that quantity divided by w_q as a signed-normalised integer
-	static LBM_DEV real ld(const elem* p, int q, real pr, real sc)
+	// stores that quantity divided by w_q as a signed-normalised integer.
+	// Loads take the inverse scale (isc = 1 / scale), formed once per site.
+	static LBM_DEV real ld(const elem* p, int q, real pr, real isc)
 	{
 		const real v = S::ld(p, (float) (w7(q) / 32767.0));
 		if constexpr (S::dev)
-			return v / sc + (real) w7(q) * pr;
+			return v * isc + (real) w7(q) * pr;
 		else
 			return v;
 	}

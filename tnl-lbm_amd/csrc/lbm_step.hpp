@@ -944,7 +944,7 @@ LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb&
 	constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
 	const int16_t gi = B.map ? B.map[s0] : (int16_t) A_PERIODIC;
 	in.gi = gi;
-	const real pr = (real) B.phi_ref, sc = (real) B.scale;
+	const real pr = (real) B.phi_ref, isc = (real) 1 / (real) B.scale;
 	auto macro_at = [&](const void* p) -> real {
 		return A.macro_f64 ? (real) ((const double*) p)[s0] : (real) ((const float*) p)[s0];
 	};
@@ -957,14 +957,23 @@ LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb&
 		Nb PX = NX;
 		if (gi == A_OUTFLOW_RIGHT)
 			PX.p = PX.o = NX.m;
+		// nontemporal hints as in the D3Q27 update: own slots and c_y = 0 pulls
+		constexpr bool NT = (LBM_NT_GENERAL >> ST) & 1;
+		auto ld = [&](const elem* p, int qs, bool nt) -> real {
+			if (nt) {
+				const elem v = ld_nt(p);
+				return K::ld(&v, qs, pr, isc);
+			}
+			return K::ld(p, qs, pr, isc);
+		};
 #pragma unroll
 		for (int q = 0; q < 7; q++) {
 			if (AA && B.even)
-				in.f[q] = K::ld(cur + q * A.XYZ + s0, q, pr, sc);
+				in.f[q] = ld(cur + q * A.XYZ + s0, q, NT);
 			else if (AA)
-				in.f[q] = K::ld(cur + opp(q) * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), pr, sc);
+				in.f[q] = ld(cur + opp(q) * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), NT && CY[q] == 0);
 			else
-				in.f[q] = K::ld(cur + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, pr, sc);
+				in.f[q] = ld(cur + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, NT && CY[q] == 0);
 		}
 	}
 }
@@ -1019,14 +1028,24 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 			}
 		}
 		elem* out = (elem*) (AA ? B.df0 : B.df1);
+		constexpr bool NT = (LBM_NT_GENERAL >> ST) & 1;
+		auto st = [&](elem* p, int qs, real v, bool nt) {
+			if (nt) {
+				elem e;
+				K::st(&e, qs, v, pr, sc);
+				st_nt(p, e);
+			}
+			else
+				K::st(p, qs, v, pr, sc);
+		};
 #pragma unroll
 		for (int q = 0; q < 7; q++) {
 			if (AA && B.even)
-				K::st(out + opp(q) * A.XYZ + s0, opp(q), f[q], pr, sc);
+				st(out + opp(q) * A.XYZ + s0, opp(q), f[q], NT);
 			else if (AA)
-				K::st(out + q * A.XYZ + sidx(NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), q, f[q], pr, sc);
+				st(out + q * A.XYZ + sidx(NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), q, f[q], NT && CY[q] == 0);
 			else
-				K::st(out + q * A.XYZ + s0, q, f[q], pr, sc);
+				st(out + q * A.XYZ + s0, q, f[q], NT);
 		}
 	}
 	if (A.out_macro && B.macro) {
@@ -1086,7 +1105,8 @@ __global__ void k_ade_export(const void* df, double* out, int64_t XYZ, int64_t n
 		return;
 #pragma unroll
 	for (int q = 0; q < 7; q++)
-		out[q * XYZ + s] = (double) K::ld((const typename K::elem*) df + q * XYZ + s, q, (typename K::real) pr, (typename K::real) sc);
+		out[q * XYZ + s] = (double) K::ld((const typename K::elem*) df + q * XYZ + s, q, (typename K::real) pr,
+										  (typename K::real) 1 / (typename K::real) sc);
 }
 
 // ===========================================================================
