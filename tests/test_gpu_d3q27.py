@@ -209,6 +209,41 @@ def test_deviation_storage_matches_oracle(storage, tol):
             assert rel_max(dg, dh) < tol, (storage, t)
 
 
+@pytest.mark.parametrize("shape", [(5, 70, 4), (3, 256, 3)])
+@pytest.mark.parametrize("pattern", ["AA", "AB"])
+@pytest.mark.parametrize("storage,tol", [("f16dev", 3e-3), ("bf16dev", 2e-2), ("i16dev", None)])
+def test_two_byte_map_kernel_matches_oracle(storage, tol, pattern, shape):
+    """The 2-byte forms through the map kernel, which addresses every slot as
+    a per-direction base plus a 32-bit offset: an all-PERIODIC map (every
+    face wraps) and a wall slab, with lanes laid over whole y-z planes
+    (Y = 70) and over whole rows (Y = 256), against the oracle.  For I16
+    every value is the oracle's or its neighbour quantum (a rounding tie
+    that FP32 contraction decides the other way); each collision spreads a
+    flip over the site's 27 outputs, so their share grows by ~2.5x a step
+    (0.2%, 0.8%, 2%, 4.7% here; the same with 64-bit addressing)."""
+    X, Y, Z = shape
+    geo = np.full((X, Y, Z), GEO.PERIODIC, dtype=np.int16)
+    geo[:, Y // 3, :] = GEO.WALL
+    g, h = pair(X, Y, Z, 0, storage=storage, pattern=pattern, geo=geo, store_range=I16_RANGE)
+    r = fields(X, Y, Z, 9, amp=0.02)
+    g.set_equilibrium_field(*r)
+    h.set_equilibrium_field(*r)
+    Wb = W[:, None, None, None]
+    for t in range(4):
+        g.step(output_macro=True)
+        h.step(output_macro=True)
+        dg = g.export_f64().cpu().numpy() - Wb
+        dh = h.export_f64() - Wb
+        if storage == "i16dev":
+            quanta = np.abs(dg - dh) / I16_QUANTUM
+            assert quanta.max() <= 1.001, (t, quanta.max())
+            assert np.mean(quanta > 0.5) < 0.01 * 3 ** t, t
+        else:
+            assert rel_max(dg, dh) < tol, (storage, pattern, t, rel_max(dg, dh))
+        for a, b in zip(g.macros(), h.macros()):
+            assert np.abs(a.cpu().numpy() - b).max() < 1e-2, t
+
+
 def split_step(lat, cuts, output_macro):
     """One A-A step of the whole lattice as several launches over y ranges (the
     A-A update is in place and each slot is read and written by one site, so

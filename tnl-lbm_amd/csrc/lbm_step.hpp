@@ -594,7 +594,13 @@ LBM_DEV void raw_moments10(const real* f, real* m)
 	m[0] = m0; m[1] = jx; m[2] = jy; m[3] = jz; m[4] = Pxx; m[5] = Pyy; m[6] = Pzz; m[7] = Pxy; m[8] = Pxz; m[9] = Pyz;
 }
 
-template <int ST, int COLL, int PATTERN, int TAPE = 0>
+// O32: every population array holds under 4 GiB (XYZ * element size <= 2^32,
+// checked on the host), so a population slot is a wave-uniform base (the
+// array of direction q, pinned in scalar registers) plus a 32-bit per-lane
+// byte offset. The site index is then formed in 32-bit arithmetic and each
+// access is one scalar-base load or store, in place of a 64-bit address per
+// lane and access.
+template <int ST, int COLL, int PATTERN, int TAPE = 0, bool O32 = false>
 struct General
 {
 	using S = Storage<ST>;
@@ -631,6 +637,25 @@ struct General
 	static LBM_DEV int64_t sidx(const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
 	{
 		return (xs * A.Zs + zs) * A.Ys + ys;
+	}
+	// slot q of the site at storage coordinates (xs, ys, zs), and of site s0
+	template <typename E>
+	static LBM_DEV E* pq(E* base, int q, const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
+	{
+		if constexpr (O32) {
+			const uint32_t i = ((uint32_t) xs * (uint32_t) A.Zs + (uint32_t) zs) * (uint32_t) A.Ys + (uint32_t) ys;
+			return at((E*) sptr<true>((char*) (base + q * A.XYZ)), i);
+		}
+		else
+			return base + q * A.XYZ + sidx(A, xs, ys, zs);
+	}
+	template <typename E>
+	static LBM_DEV E* pq0(E* base, int q, const StepArgs& A, int64_t s0)
+	{
+		if constexpr (O32)
+			return at((E*) sptr<true>((char*) (base + q * A.XYZ)), (uint32_t) s0);
+		else
+			return base + q * A.XYZ + s0;
 	}
 
 	static LBM_DEV void set_eq(real* f, real rho, real vx, real vy, real vz)
@@ -703,18 +728,18 @@ struct General
 			else if (AA && A.even) {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = ldq(cur + q * A.XYZ + sidx(A, PX.o, NY.o, NZ.o), fx_sc(q, A.rs), NT);
+					f[q] = ldq(pq(cur, q, A, PX.o, NY.o, NZ.o), fx_sc(q, A.rs), NT);
 			}
 			else if (AA) {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = ldq(cur + opp(q) * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(opp(q), A.rs),
+					f[q] = ldq(pq(cur, opp(q), A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(opp(q), A.rs),
 							   NT && CY[q] == 0);
 			}
 			else {
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = ldq(cur + q * A.XYZ + sidx(A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs), NT && CY[q] == 0);
+					f[q] = ldq(pq(cur, q, A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), fx_sc(q, A.rs), NT && CY[q] == 0);
 			}
 
 			// the raw moments of the pulled populations, recorded after the step's own stores
@@ -875,19 +900,19 @@ struct General
 				elem* __restrict__ out = (elem*) A.df0;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					stq(out + opp(q) * A.XYZ + s0, f[q], fx_inv(opp(q), A.invr), NT);
+					stq(pq0(out, opp(q), A, s0), f[q], fx_inv(opp(q), A.invr), NT);
 			}
 			else if (AA) {
 				elem* __restrict__ out = (elem*) A.df0;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					stq(out + q * A.XYZ + sidx(A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q], fx_inv(q, A.invr), NT && CY[q] == 0);
+					stq(pq(out, q, A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), f[q], fx_inv(q, A.invr), NT && CY[q] == 0);
 			}
 			else {
 				elem* __restrict__ out = (elem*) A.df1;
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					stq(out + q * A.XYZ + s0, f[q], fx_inv(q, A.invr), NT);
+					stq(pq0(out, q, A, s0), f[q], fx_inv(q, A.invr), NT);
 			}
 			if constexpr (TAPE != 0) {
 				if (rec) {
@@ -909,14 +934,14 @@ struct General
 // The FP64 default cumulant needs ~170 VGPRs here: asking for 3 waves per SIMD
 // (<= 168) costs at most 2 spilled registers; the alternative operators keep
 // their registers.
-template <int ST, int COLL, int PATTERN, int TAPE = 0>
+template <int ST, int COLL, int PATTERN, int TAPE = 0, bool O32 = false>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(COLL == TNL_LBM_COLL_CUM ? 3 : 1, 8))) k_general(const StepArgs A)
 {
 	int64_t x, z, y;
 	if (! block_coords(A, x, z, y))
 		return;
-	General<ST, COLL, PATTERN, TAPE>::site(A, x, y, z);
+	General<ST, COLL, PATTERN, TAPE, O32>::site(A, x, y, z);
 }
 
 // ===========================================================================
@@ -1341,7 +1366,21 @@ int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, 
 	else {
 		StepArgs B = A;
 		linearize(B, threads, grid);
-		if (cfg->pattern == TNL_LBM_PATTERN_AA)
+		// 2-byte forms: 32-bit slot offsets where a population array is under
+		// 4 GiB (F16 384^3 channel: A-A 1821 -> 1430 us, 139 -> 106 VGPRs;
+		// A-B -2%; neutral for the 4- and 8-byte forms, which keep 64-bit)
+		bool o32 = false;
+		if constexpr (sizeof(typename Storage<ST>::elem) == 2)
+			o32 = (uint64_t) A.XYZ * sizeof(typename Storage<ST>::elem) <= (1ull << 32);
+		if (o32) {
+			if constexpr (sizeof(typename Storage<ST>::elem) == 2) {
+				if (cfg->pattern == TNL_LBM_PATTERN_AA)
+					hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA, 0, true>), dim3(grid), dim3(threads), 0, st, B);
+				else
+					hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB, 0, true>), dim3(grid), dim3(threads), 0, st, B);
+			}
+		}
+		else if (cfg->pattern == TNL_LBM_PATTERN_AA)
 			hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, B);
 		else
 			hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, B);
