@@ -313,7 +313,18 @@ int tnl_lbm_nse_ade_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* ns
 	linearize(A, threads, grid);
 	auto go = [&](auto stc) {
 		constexpr int ST = decltype(stc)::value;
-		if (cfg->pattern == TNL_LBM_PATTERN_AA)
+		// 2-byte forms: 32-bit slot offsets where each population array (D3Q27
+		// and D3Q7 share XYZ) holds at most 4 GiB, as in the map kernel
+		constexpr bool two = sizeof(typename Storage<ST>::elem) == 2;
+		if (two && (uint64_t) A.XYZ * 2 <= (1ull << 32)) {
+			if constexpr (two) {
+				if (cfg->pattern == TNL_LBM_PATTERN_AA)
+					hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA, true>), dim3(grid), dim3(threads), 0, st, A, B);
+				else
+					hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, true>), dim3(grid), dim3(threads), 0, st, A, B);
+			}
+		}
+		else if (cfg->pattern == TNL_LBM_PATTERN_AA)
 			hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, A, B);
 		else
 			hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, A, B);

@@ -594,6 +594,29 @@ LBM_DEV void raw_moments10(const real* f, real* m)
 	m[0] = m0; m[1] = jx; m[2] = jy; m[3] = jz; m[4] = Pxx; m[5] = Pyy; m[6] = Pzz; m[7] = Pxy; m[8] = Pxz; m[9] = Pyz;
 }
 
+// Slot q of the site at storage coordinates (xs, ys, zs), and of site s:
+// with O32, the array of direction q as a wave-uniform base (pinned in
+// scalar registers) plus a 32-bit per-lane byte offset formed in 32-bit
+// arithmetic; else a 64-bit address per lane.
+template <bool O32, typename E>
+LBM_DEV E* slot(E* base, int q, const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
+{
+	if constexpr (O32) {
+		const uint32_t i = ((uint32_t) xs * (uint32_t) A.Zs + (uint32_t) zs) * (uint32_t) A.Ys + (uint32_t) ys;
+		return at((E*) sptr<true>((char*) (base + q * A.XYZ)), i);
+	}
+	else
+		return base + q * A.XYZ + (xs * A.Zs + zs) * A.Ys + ys;
+}
+template <bool O32, typename E>
+LBM_DEV E* slot0(E* base, int q, const StepArgs& A, int64_t s)
+{
+	if constexpr (O32)
+		return at((E*) sptr<true>((char*) (base + q * A.XYZ)), (uint32_t) s);
+	else
+		return base + q * A.XYZ + s;
+}
+
 // O32: every population array holds under 4 GiB (XYZ * element size <= 2^32,
 // checked on the host), so a population slot is a wave-uniform base (the
 // array of direction q, pinned in scalar registers) plus a 32-bit per-lane
@@ -638,24 +661,15 @@ struct General
 	{
 		return (xs * A.Zs + zs) * A.Ys + ys;
 	}
-	// slot q of the site at storage coordinates (xs, ys, zs), and of site s0
 	template <typename E>
 	static LBM_DEV E* pq(E* base, int q, const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
 	{
-		if constexpr (O32) {
-			const uint32_t i = ((uint32_t) xs * (uint32_t) A.Zs + (uint32_t) zs) * (uint32_t) A.Ys + (uint32_t) ys;
-			return at((E*) sptr<true>((char*) (base + q * A.XYZ)), i);
-		}
-		else
-			return base + q * A.XYZ + sidx(A, xs, ys, zs);
+		return slot<O32>(base, q, A, xs, ys, zs);
 	}
 	template <typename E>
 	static LBM_DEV E* pq0(E* base, int q, const StepArgs& A, int64_t s0)
 	{
-		if constexpr (O32)
-			return at((E*) sptr<true>((char*) (base + q * A.XYZ)), (uint32_t) s0);
-		else
-			return base + q * A.XYZ + s0;
+		return slot0<O32>(base, q, A, s0);
 	}
 
 	static LBM_DEV void set_eq(real* f, real rho, real vx, real vy, real vz)
@@ -959,7 +973,7 @@ struct AdeIn
 	typename Storage<ST>::real D, S;
 };
 
-template <int ST, int PATTERN>
+template <int ST, int PATTERN, bool O32 = false>
 LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb& NX, const Nb& NY, const Nb& NZ,
 					  AdeIn<ST>& in)
 {
@@ -978,7 +992,6 @@ LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb&
 	in.S = B.source ? macro_at(B.source) : (real) 0;
 	if (gi != A_NOTHING) {
 		const elem* cur = (const elem*) B.df0;
-		auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
 		Nb PX = NX;
 		if (gi == A_OUTFLOW_RIGHT)
 			PX.p = PX.o = NX.m;
@@ -994,16 +1007,16 @@ LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb&
 #pragma unroll
 		for (int q = 0; q < 7; q++) {
 			if (AA && B.even)
-				in.f[q] = ld(cur + q * A.XYZ + s0, q, NT);
+				in.f[q] = ld(slot0<O32>(cur, q, A, s0), q, NT);
 			else if (AA)
-				in.f[q] = ld(cur + opp(q) * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), NT && CY[q] == 0);
+				in.f[q] = ld(slot<O32>(cur, opp(q), A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), opp(q), NT && CY[q] == 0);
 			else
-				in.f[q] = ld(cur + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, NT && CY[q] == 0);
+				in.f[q] = ld(slot<O32>(cur, q, A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]), q, NT && CY[q] == 0);
 		}
 	}
 }
 
-template <int ST, int PATTERN>
+template <int ST, int PATTERN, bool O32 = false>
 LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename Storage<ST>::real vx,
 					  typename Storage<ST>::real vy, typename Storage<ST>::real vz, const Nb& NX, const Nb& NY, const Nb& NZ,
 					  const AdeIn<ST>& in)
@@ -1017,7 +1030,6 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 	const real D = in.D;
 	real phi = 0;
 	if (gi != A_NOTHING) {
-		auto sidx = [&](int64_t xs, int64_t ys, int64_t zs) { return (xs * A.Zs + zs) * A.Ys + ys; };
 		real f[7];
 #pragma unroll
 		for (int q = 0; q < 7; q++)
@@ -1066,11 +1078,11 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 #pragma unroll
 		for (int q = 0; q < 7; q++) {
 			if (AA && B.even)
-				st(out + opp(q) * A.XYZ + s0, opp(q), f[q], NT);
+				st(slot0<O32>(out, opp(q), A, s0), opp(q), f[q], NT);
 			else if (AA)
-				st(out + q * A.XYZ + sidx(NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), q, f[q], NT && CY[q] == 0);
+				st(slot<O32>(out, q, A, NX[CX[q]], NY[CY[q]], NZ[CZ[q]]), q, f[q], NT && CY[q] == 0);
 			else
-				st(out + q * A.XYZ + s0, q, f[q], NT);
+				st(slot0<O32>(out, q, A, s0), q, f[q], NT);
 		}
 	}
 	if (A.out_macro && B.macro) {
@@ -1081,7 +1093,7 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 	}
 }
 
-template <int ST, int COLL, int PATTERN>
+template <int ST, int COLL, int PATTERN, bool O32 = false>
 __global__ void __launch_bounds__(256) k_nse_ade(const StepArgs A, const AdeArgs B)
 {
 	int64_t x, z, y;
@@ -1095,11 +1107,11 @@ __global__ void __launch_bounds__(256) k_nse_ade(const StepArgs A, const AdeArgs
 	const Nb NY = neighbours<AA>(y, A.Y, A.oy, A.dist_y, per);
 	const Nb NZ = neighbours<AA>(z, A.Z, A.oz, A.dist_z, per);
 	AdeIn<ST> in;
-	ade_load<ST, PATTERN>(A, B, s0, NX, NY, NZ, in);
+	ade_load<ST, PATTERN, O32>(A, B, s0, NX, NY, NZ, in);
 	typename Storage<ST>::real vx, vy, vz;
 	Nb GX, GY, GZ;
-	General<ST, COLL, PATTERN>::site(A, x, y, z, vx, vy, vz, GX, GY, GZ);
-	ade_site<ST, PATTERN>(A, B, s0, vx, vy, vz, NX, NY, NZ, in);
+	General<ST, COLL, PATTERN, 0, O32>::site(A, x, y, z, vx, vy, vz, GX, GY, GZ);
+	ade_site<ST, PATTERN, O32>(A, B, s0, vx, vy, vz, NX, NY, NZ, in);
 }
 
 template <int ST>
