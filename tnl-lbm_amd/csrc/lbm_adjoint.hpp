@@ -295,7 +295,7 @@ LBM_DEV bool shell_coords(const StepArgs& A, int64_t& x, int64_t& z, int64_t& y)
 // sources, after the first pass.
 // (256, 2): two waves per SIMD; the FP64 form then spills a few registers
 // instead of running one wave per SIMD at 264
-template <int ST, bool SPECIAL, int TAPE = 0>
+template <int ST, bool SPECIAL, int TAPE = 0, bool O32 = false>
 __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const AdjPtrs P)
 {
 	using S = Storage<ST>;
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 		real lam[27];
 #pragma unroll
 		for (int q = 0; q < 27; q++)
-			lam[q] = S::ld(Ln + q * A.XYZ + s0);
+			lam[q] = S::ld(slot0<O32, false>(Ln, q, A, s0));
 		// raw moments of the pulled populations: from the state, or recorded by the forward step
 		const bool needs_m = gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_EQ;
 		real m[D::NMOM];
@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 				real f[27];
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					f[q] = S::ld(F + q * A.XYZ + sidx(PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
+					f[q] = S::ld(slot<O32, false>(F, q, A, PX[-CX[q]], NY[-CY[q]], NZ[-CZ[q]]));
 				D::raw_moments(f, m);
 			}
 			else {
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 #pragma unroll
 		for (int q = 0; q < 27; q++) {
 			const real v = (irregular >> q) & 1u && ! unsupported ? (real) 0 : mu[q];
-			S::st(L + q * A.XYZ + sidx(WX[-CX[q]], WY[-CY[q]], WZ[-CZ[q]]), v);
+			S::st(slot<O32, false>(L, q, A, WX[-CX[q]], WY[-CY[q]], WZ[-CZ[q]]), v);
 		}
 	}
 	else {

@@ -595,24 +595,26 @@ LBM_DEV void raw_moments10(const real* f, real* m)
 }
 
 // Slot q of the site at storage coordinates (xs, ys, zs), and of site s:
-// with O32, the array of direction q as a wave-uniform base (pinned in
-// scalar registers) plus a 32-bit per-lane byte offset formed in 32-bit
-// arithmetic; else a 64-bit address per lane.
-template <bool O32, typename E>
+// with O32, the array of direction q as a wave-uniform base plus a 32-bit
+// per-lane byte offset formed in 32-bit arithmetic; else a 64-bit address
+// per lane. PIN keeps the base in scalar registers (one scalar-base access
+// each); it needs the call to sit where the compiler can prove the base
+// uniform, which the adjoint's branchy body does not allow.
+template <bool O32, bool PIN = true, typename E>
 LBM_DEV E* slot(E* base, int q, const StepArgs& A, int64_t xs, int64_t ys, int64_t zs)
 {
 	if constexpr (O32) {
 		const uint32_t i = ((uint32_t) xs * (uint32_t) A.Zs + (uint32_t) zs) * (uint32_t) A.Ys + (uint32_t) ys;
-		return at((E*) sptr<true>((char*) (base + q * A.XYZ)), i);
+		return at((E*) sptr<PIN>((char*) (base + q * A.XYZ)), i);
 	}
 	else
 		return base + q * A.XYZ + (xs * A.Zs + zs) * A.Ys + ys;
 }
-template <bool O32, typename E>
+template <bool O32, bool PIN = true, typename E>
 LBM_DEV E* slot0(E* base, int q, const StepArgs& A, int64_t s)
 {
 	if constexpr (O32)
-		return at((E*) sptr<true>((char*) (base + q * A.XYZ)), (uint32_t) s);
+		return at((E*) sptr<PIN>((char*) (base + q * A.XYZ)), (uint32_t) s);
 	else
 		return base + q * A.XYZ + s;
 }
