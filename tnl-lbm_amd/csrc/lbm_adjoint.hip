@@ -37,7 +37,12 @@ int record_for(const StepArgs& A0, unsigned threads, unsigned grid, hipStream_t 
 {
 	StepArgs A = A0;
 	linearize(A, threads, grid);
-	hipLaunchKernelGGL((k_general<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, TAPE>), dim3(grid), dim3(threads), 0, st, A);
+	// 32-bit slot offsets where each population array holds at most 4 GiB:
+	// -124 (FP64) / -151 (FP32) VALU, ~4% on the C4 duct (kernel trace)
+	if ((uint64_t) A.XYZ * sizeof(typename Storage<ST>::elem) <= (1ull << 32))
+		hipLaunchKernelGGL((k_general<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, TAPE, true>), dim3(grid), dim3(threads), 0, st, A);
+	else
+		hipLaunchKernelGGL((k_general<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, TAPE>), dim3(grid), dim3(threads), 0, st, A);
 	return hip_check(hipGetLastError(), "recording step launch");
 }
 
