@@ -19,9 +19,12 @@
 // rate, the defining relations of the boundary codes INFLOW_LEFT,
 // INFLOW_BB_LEFT, OUTFLOW_RIGHT_INTERP, SYM_* and the D3Q7 INFLOW / SYM /
 // OUTFLOW_RIGHT codes, the KBC stabiliser as the stationary point of the
-// quadratic entropy, the 2017 limiter's form and lambda, the adjoint by
-// dot-product identities.  Partially pinned (through invariants only): the
-// 2017 rate formulas omega3-5 and the coefficients A, B.  The reference has no golden
+// quadratic entropy, the adjoint by dot-product identities, and the three
+// cumulant collisions in cumulant space: the cumulants before and after a
+// collision, computed independently as derivatives of the log of the discrete
+// moment-generating function, obey the reference's relaxation rules (orders
+// 2..6, the 2017 limiter with the closed-form rates read back per channel, A
+// and B through the fourth-order cumulants).  The reference has no golden
 // vectors for this path (its only test fixtures are unrelated IBM matrices
 // stored as Git-LFS pointers).
 

@@ -41,6 +41,25 @@ MUTANTS = {
     "relaxation_rate": ("const dreal omega1 = no1 / (no3 * KS.lbmViscosity + n1o2);",
                         "const dreal omega1 = no1 / (no3 * KS.lbmViscosity + no1);",
                         "test_oracle_lattice.py", "shear_wave"),
+    # the 2017 rates and coefficients, and the default variant's product terms
+    "cum2017_w3_on_w4_channel": ("const dreal omega120m102 = omega4 + (no1 - omega4) * std::fabs(C_120 - C_102)",
+                                 "const dreal omega120m102 = omega3 + (no1 - omega3) * std::fabs(C_120 - C_102)",
+                                 "test_oracle_cumulant_space.py", "closed_forms_per_channel"),
+    "cum2017_w5_coefficient": ("omega1 * omega2 * (no18 - no13 * omega1)", "omega1 * omega2 * (no18 - (dreal) 12 * omega1)",
+                               "test_oracle_cumulant_space.py", "closed_forms_per_channel"),
+    "cum2017_A_coefficient": ("omega2 * omega2 * (omega1 * (no10 - no3 * omega1) - no4))",
+                              "omega2 * omega2 * (omega1 * (no10 - no2 * omega1) - no4))",
+                              "test_oracle_cumulant_space.py", "relaxation_rules"),
+    "cum2017_B_coefficient": ("(no4 * omega1 * omega2 * (no9 * omega1 - no16)", "(no4 * omega1 * omega2 * (no8 * omega1 - no16)",
+                              "test_oracle_cumulant_space.py", "relaxation_rules"),
+    "cum2017_eq44_symmetrised": ("A * rho * (Dxu + Dyv - no2 * Dzw)", "A * rho * (Dyv + Dzw - no2 * Dxu)",
+                                 "test_oracle_cumulant_space.py", "relaxation_rules"),
+    "cum2017_eq83_index": ("Cs_122 + (ks_020 * ks_102 + ks_002 * ks_120 + no4 * ks_011 * ks_111",
+                           "Cs_122 + (ks_020 * ks_102 + ks_002 * ks_120 + no4 * ks_101 * ks_111",
+                           "test_oracle_cumulant_space.py", "relaxation_rules"),
+    "cum_default_cs110_squared": ("const dreal ks_220 = (ks_020 * ks_200 + no2 * ks_110 * ks_110) * rho_inv;",
+                                  "const dreal ks_220 = (ks_020 * ks_200 + ks_110 * ks_110) * rho_inv;",
+                                  "test_oracle_cumulant_space.py", "relaxation_rules"),
 }
 
 

@@ -9,6 +9,7 @@ import torch
 
 import oracle as O
 from lbm_common import C, CX, CY, CZ, OPP, W, cube_symmetries, perm_of
+from test_oracle_cumulant_space import w3_of, w4_of, w5_of
 
 CUMS = [0, 1, 2]  # CUM, CUM_2017, CUM_2017_ANTIALIAS
 
@@ -229,9 +230,11 @@ def moment_dual(i, j, k):
     return np.linalg.solve(E, e)
 
 
-# third-order combinations with their own limiter (col_cum.h:198-227)
+# third-order combinations with their own limiter (col_cum.h:198-227) and the
+# closed-form rate of each channel (Geier et al. 2017 at w2 = 1)
 LIMITED = {"120+102": (((1, 2, 0), 0.5), ((1, 0, 2), 0.5)), "210-012": (((2, 1, 0), 0.5), ((0, 1, 2), -0.5)),
            "111": (((1, 1, 1), 1.0),)}
+CHANNEL_RATE = {"120+102": w3_of, "210-012": w4_of, "111": w5_of}
 
 
 @pytest.mark.parametrize("coll", [1, 2])
@@ -243,7 +246,9 @@ def test_2017_limiter_form_and_lambda(coll, combo):
     inverse is linear in |C| with intercept / (slope rho) = lambda = 0.01.
     At u = 0 a perturbation carrying only this combination leaves the other
     cumulants as they were, and order-3 cumulants are the raw moments.  rho
-    differs from 1 so that rho lambda and lambda are told apart."""
+    differs from 1 so that rho lambda and lambda are told apart.  The intercept
+    gives the channel's rate w_n, which must be the closed form for that
+    channel (w3 for C_120 + C_102, w4 for C_210 - C_012, w5 for C_111)."""
     rho, nu = 1.05, 0.01
     feq = O.equilibrium(rho, 0.0, 0.0, 0.0, kind=0)
     h = sum(wt * moment_dual(*m) for m, wt in LIMITED[combo])
@@ -264,4 +269,5 @@ def test_2017_limiter_form_and_lambda(coll, combo):
     assert np.abs(I + S * amps - inv).max() < 1e-9 * np.abs(inv).max()
     assert abs(I / (S * rho) / 0.01 - 1) < 1e-8
     omega_n = 1 - 1 / I
-    assert 0 < omega_n < 2
+    expect = CHANNEL_RATE[combo](1 / (3 * nu + 0.5))
+    assert abs(omega_n - expect) < 1e-8, (omega_n, expect)
