@@ -177,16 +177,18 @@ typedef struct tnl_lbm_d3q27_data {
 	int64_t ox, oy, oz;   /* overlaps (indexer overlaps; 0 on one GPU)      */
 	int64_t XYZ;          /* population stride; >= (X+2ox)(Y+2oy)(Z+2oz)    */
 	int32_t even_iter;    /* A-A parity, LBM::updateKernelData (lbm.hpp:311) */
-	int32_t reserved0;
+	int32_t inflow_profile_real; /* enum tnl_lbm_real: element type of inflow_profile
+	                                (F64 = 0, the default; F32 for TraitsSP's dreal) */
 	double lbm_viscosity; /* LBM_Data::lbmViscosity                          */
 	double fx, fy, fz;    /* NSE_Data homogeneous force (lbm_data.h:69-72)   */
 	void* df[2];          /* dfs[df_cur], dfs[df_out] (df_out: A-B only)     */
 	void* macro;          /* dmacro: 4*XYZ elements of macro_real, or NULL   */
 	const int16_t* map;   /* dmap: XYZ geometry codes; NULL = all PERIODIC   */
 	double inflow_vx, inflow_vy, inflow_vz; /* NSE_Data_ConstInflow (lbm_data.h:81-91) */
-	/* NSE_Data_InflowProfile (lbm_data.h:100-132): double arrays indexed
-	 * Y*z + y; a non-NULL pointer overrides the constant component.        */
-	const double* inflow_profile[3];
+	/* NSE_Data_InflowProfile (lbm_data.h:100-132): device arrays of
+	 * inflow_profile_real elements indexed Y*z + y; a non-NULL pointer
+	 * overrides the constant component.                                    */
+	const void* inflow_profile[3];
 	double store_range;   /* I16_DEV only: R, the bound of |f - w_q| / w_q (> 0) */
 	/* I16_DEV only: the bound per direction class |c_q|^2 = 0, 1, 2, 3 (rest,
 	 * faces, edges, corners); a class with 0 uses store_range.  The classes'

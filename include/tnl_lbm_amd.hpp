@@ -11,6 +11,12 @@
  *     ...
  *     tnl_lbm_amd::step<NSE>(block.data, offset, end, block.is_distributed(), compute_macro, stream);
  *
+ * and the coupled update of State_NSE_ADE (state_NSE_ADE.h:201-210) the same
+ * way:
+ *
+ *     tnl_lbm_amd::step_coupled<NSE, ADE>(block_nse.data, block_ade.data, offset, end,
+ *                                         block_nse.is_distributed(), compute_macro, stream);
+ *
  * Mapped types (forward-declared below with the reference's own template
  * parameter lists, lbm3d/d3q27/col_*.h; declared without default arguments so
  * that the reference's headers may come before or after this one):
@@ -22,7 +28,12 @@
  *   D3Q27_KBC_N1..N4, D3Q27_KBC_C1..C4 -> TNL_LBM_COLL_KBC_*
  *   D3Q27_STREAMING                  -> A-A under AA_PATTERN, else A-B
  *                                       (core.h:20-25)
- *   TRAITS::dreal double / float     -> F64 / F32 storage and macro arrays
+ *   TRAITS::dreal double / float     -> F64 / F32 storage, macro and inflow
+ *                                       profile arrays
+ *   D3Q7_SRT / D3Q7_MRT / D3Q7_CLBM  -> TNL_LBM_ADE_COLL_SRT / MRT / CLBM
+ *                                       (d3q7/col_srt.h, col_mrt.h, col_clbm.h)
+ * LBM_Data::dfs has DFMAX entries (defs.h:45-66): one under AA_PATTERN, and
+ * dfs[1] (df_out) is read only where the array has a second entry.
  * The deviation storage forms (F32_DEV ... I16_DEV) have no reference type:
  * pass the storage explicitly.  An unsupported collision fails to compile.
  */
@@ -53,6 +64,12 @@ template <typename TRAITS, typename LBM_EQ>
 struct D3Q27_KBC_C3;
 template <typename TRAITS, typename LBM_EQ>
 struct D3Q27_KBC_C4;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q7_SRT;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q7_MRT;
+template <typename TRAITS, typename LBM_EQ>
+struct D3Q7_CLBM;
 
 namespace tnl_lbm_amd {
 
@@ -85,6 +102,18 @@ template <typename T, typename E> struct collision_id<D3Q27_KBC_C1<T, E>> { stat
 template <typename T, typename E> struct collision_id<D3Q27_KBC_C2<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_C2; };
 template <typename T, typename E> struct collision_id<D3Q27_KBC_C3<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_C3; };
 template <typename T, typename E> struct collision_id<D3Q27_KBC_C4<T, E>> { static constexpr int value = TNL_LBM_COLL_KBC_C4; };
+// clang-format on
+
+// the D3Q7 operator of an ADE configuration (ADE::COLL)
+template <typename COLL>
+struct ade_collision_id
+{
+	static_assert(sizeof(COLL) == 0, "this D3Q7 collision operator has no counterpart in libtnl_lbm_amd");
+};
+// clang-format off
+template <typename T, typename E> struct ade_collision_id<D3Q7_SRT<T, E>>  { static constexpr int value = TNL_LBM_ADE_COLL_SRT; };
+template <typename T, typename E> struct ade_collision_id<D3Q7_MRT<T, E>>  { static constexpr int value = TNL_LBM_ADE_COLL_MRT; };
+template <typename T, typename E> struct ade_collision_id<D3Q7_CLBM<T, E>> { static constexpr int value = TNL_LBM_ADE_COLL_CLBM; };
 // clang-format on
 
 // the streaming plugin the reference compiled in (core.h:20-25)
@@ -129,6 +158,24 @@ struct has_force : std::false_type
 template <typename D>
 struct has_force<D, std::void_t<decltype(std::declval<D&>().fx)>> : std::true_type
 {};
+template <typename D, typename = void>
+struct has_inflow_phi : std::false_type
+{};
+template <typename D>
+struct has_inflow_phi<D, std::void_t<decltype(std::declval<D&>().inflow_phi)>> : std::true_type
+{};
+
+// dfs[0] and, where LBM_Data::dfs has a second entry (DFMAX > 1, defs.h:45-66), dfs[1]
+template <typename DATA>
+inline void copy_dfs(DATA& D, void* df[2])
+{
+	using arr = std::remove_reference_t<decltype(D.dfs)>;
+	df[0] = D.dfs[0];
+	if constexpr (std::extent<arr>::value > 1)
+		df[1] = D.dfs[1];
+	else
+		df[1] = nullptr;
+}
 }  // namespace detail
 
 // tnl_lbm_d3q27_data of a block's NSE::DATA (LBM_Data, NSE_Data,
@@ -153,16 +200,17 @@ inline tnl_lbm_d3q27_data data_for(DATA& D)
 		sd.fy = D.fy;
 		sd.fz = D.fz;
 	}
-	sd.df[0] = D.dfs[0];
-	sd.df[1] = D.dfs[1];
+	detail::copy_dfs(D, sd.df);
 	sd.macro = D.dmacro;
 	sd.map = D.dmap;
 	if constexpr (detail::has_inflow<DATA>::value) {
 		using V = std::remove_reference_t<decltype(D.inflow_vx)>;
 		if constexpr (std::is_pointer<V>::value) {
-			// NSE_Data_InflowProfile: per-(y, z) arrays indexed Y*z + y
-			static_assert(std::is_same<std::remove_cv_t<std::remove_pointer_t<V>>, double>::value,
-						  "inflow profiles are passed as double arrays");
+			// NSE_Data_InflowProfile: per-(y, z) arrays of dreal indexed Y*z + y
+			using E = std::remove_cv_t<std::remove_pointer_t<V>>;
+			static_assert(std::is_same<E, double>::value || std::is_same<E, float>::value,
+						  "inflow profiles must be double or float arrays");
+			sd.inflow_profile_real = std::is_same<E, float>::value ? TNL_LBM_REAL_F32 : TNL_LBM_REAL_F64;
 			sd.inflow_profile[0] = D.inflow_vx;
 			sd.inflow_profile[1] = D.inflow_vy;
 			sd.inflow_profile[2] = D.inflow_vz;
@@ -174,6 +222,40 @@ inline tnl_lbm_d3q27_data data_for(DATA& D)
 		}
 	}
 	return sd;
+}
+
+// tnl_lbm_d3q7_data of a block's ADE::DATA (ADE_Data or ADE_Data_ConstInflow,
+// lbm_data.h:212-261) for the D3Q7 operator ADE_COLL.  The reference has no
+// source term (a TODO at lbm_data.h:229): `source` stays NULL.  phi_ref and
+// store_scale matter for the deviation storage forms only.
+template <typename ADE_COLL, typename DATA>
+inline tnl_lbm_d3q7_data ade_data_for(DATA& D, double phi_ref = 0.0, double store_scale = 1.0)
+{
+	tnl_lbm_d3q7_data ad{};
+	ad.struct_size = sizeof ad;
+	ad.version = TNL_LBM_ABI_VERSION;
+	ad.X = D.X();
+	ad.Y = D.Y();
+	ad.Z = D.Z();
+	ad.ox = D.indexer.template getOverlap<0>();
+	ad.oy = D.indexer.template getOverlap<1>();
+	ad.oz = D.indexer.template getOverlap<2>();
+	ad.XYZ = D.XYZ;
+	ad.even_iter = D.even_iter ? 1 : 0;
+	ad.collision = ade_collision_id<ADE_COLL>::value;
+	ad.lbm_diffusivity = D.lbmViscosity;  // ADE_Data::diffusionCoefficient (lbm_data.h:233-242)
+	detail::copy_dfs(D, ad.df);
+	ad.macro = D.dmacro;
+	ad.map = D.dmap;
+	ad.diffusion_coefficient = D.diffusion_coefficient_ptr;
+	ad.source = nullptr;
+	if constexpr (detail::has_inflow_phi<DATA>::value)
+		ad.inflow_phi = D.inflow_phi;
+	else
+		ad.inflow_phi = 1.0;  // ADE_Data_ConstInflow's default (lbm_data.h:252)
+	ad.phi_ref = phi_ref;
+	ad.store_scale = store_scale;
+	return ad;
 }
 
 // flags of the reference's call: distributed (x, y, z) and output_macro
@@ -202,6 +284,25 @@ inline int step(DATA& SD, const IDX3D& offset, const IDX3D& end, const DIST& dis
 	const int64_t b[3] = {(int64_t) offset.x(), (int64_t) offset.y(), (int64_t) offset.z()};
 	const int64_t e[3] = {(int64_t) end.x(), (int64_t) end.y(), (int64_t) end.z()};
 	return tnl_lbm_d3q27_step(&cfg, &sd, b, e, flags_for(distributed, output_macro), stream);
+}
+
+// One launch of cudaLBMKernel<NSE, ADE>(nse, ade, offset, end, distributed,
+// output_macro) (kernels.h:114-196, launched at state_NSE_ADE.h:201-210):
+// the D3Q27 update and the D3Q7 update at the same site.  Both configurations
+// must share the dreal type; the storage form is the NSE one.
+template <typename NSE, typename ADE, typename NDATA, typename ADATA, typename IDX3D, typename DIST>
+inline int step_coupled(NDATA& nse, ADATA& ade, const IDX3D& offset, const IDX3D& end, const DIST& distributed,
+						bool output_macro, void* stream = nullptr, int storage = -1, double phi_ref = 0.0,
+						double store_scale = 1.0)
+{
+	static_assert(std::is_same<typename NSE::TRAITS::dreal, typename ADE::TRAITS::dreal>::value,
+				  "the coupled kernel takes NSE and ADE arrays of one element type");
+	const tnl_lbm_config cfg = config_for<NSE>(storage);
+	const tnl_lbm_d3q27_data sd = data_for(nse);
+	const tnl_lbm_d3q7_data ad = ade_data_for<typename ADE::COLL>(ade, phi_ref, store_scale);
+	const int64_t b[3] = {(int64_t) offset.x(), (int64_t) offset.y(), (int64_t) offset.z()};
+	const int64_t e[3] = {(int64_t) end.x(), (int64_t) end.y(), (int64_t) end.z()};
+	return tnl_lbm_nse_ade_step(&cfg, &sd, &ad, b, e, flags_for(distributed, output_macro), stream);
 }
 
 }  // namespace tnl_lbm_amd

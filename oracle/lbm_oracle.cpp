@@ -77,6 +77,19 @@ int dir_of(int cx, int cy, int cz)
 	return -1;
 }
 
+// inflow velocity component k at profile index i = Y z + y: the profile
+// array's element (double or float, inflow_profile_real), else the constant
+// (NSE_Data_InflowProfile::vel_*, NSE_Data_ConstInflow, lbm_data.h:81-125)
+double inflow_at(const tnl_lbm_d3q27_data* SD, int k, int64_t i)
+{
+	const double c[3] = {SD->inflow_vx, SD->inflow_vy, SD->inflow_vz};
+	if (! SD->inflow_profile[k])
+		return c[k];
+	if (SD->inflow_profile_real == TNL_LBM_REAL_F32)
+		return (double) static_cast<const float*>(SD->inflow_profile[k])[i];
+	return static_cast<const double*>(SD->inflow_profile[k])[i];
+}
+
 // rest weights w_q = W(cx) W(cy) W(cz), W(0)=2/3, W(+-1)=1/6
 double w27(int q)
 {
@@ -1272,9 +1285,9 @@ struct Site27
 	{
 		// NSE_Data_ConstInflow::inflow / NSE_Data_InflowProfile::inflow (lbm_data.h:85-132)
 		const int64_t i = SD->Y * z + y;
-		KS.vx = (real) (SD->inflow_profile[0] ? SD->inflow_profile[0][i] : SD->inflow_vx);
-		KS.vy = (real) (SD->inflow_profile[1] ? SD->inflow_profile[1][i] : SD->inflow_vy);
-		KS.vz = (real) (SD->inflow_profile[2] ? SD->inflow_profile[2][i] : SD->inflow_vz);
+		KS.vx = (real) inflow_at(SD, 0, i);
+		KS.vy = (real) inflow_at(SD, 1, i);
+		KS.vz = (real) inflow_at(SD, 2, i);
 	}
 
 	static bool supported_geo(int16_t g)
@@ -2074,9 +2087,9 @@ int step(const tnl_lbm_d3q27_data* SD, const void* lam_next, void* lam, double* 
 				}
 				const int64_t iy = SD->Y * z + y;
 				Dual uin[3];
-				uin[0] = Dual(SD->inflow_profile[0] ? SD->inflow_profile[0][iy] : SD->inflow_vx);
-				uin[1] = Dual(SD->inflow_profile[1] ? SD->inflow_profile[1][iy] : SD->inflow_vy);
-				uin[2] = Dual(SD->inflow_profile[2] ? SD->inflow_profile[2][iy] : SD->inflow_vz);
+				uin[0] = Dual(inflow_at(SD, 0, iy));
+				uin[1] = Dual(inflow_at(SD, 1, iy));
+				uin[2] = Dual(inflow_at(SD, 2, iy));
 				for (int d = 0; d < 3; d++)
 					uin[d].d[27 + d] = 1;
 				site_map(gi, KS, uin, SD->lbm_viscosity, SD->fx, SD->fy, SD->fz);

@@ -143,12 +143,15 @@ class HostLattice:
     def from_storage(self, v):
         return v.reshape(self.X, self.Z, self.Y).transpose(0, 2, 1)
 
-    def set_inflow(self, vx=0.0, vy=0.0, vz=0.0, profile=None):
+    def set_inflow(self, vx=0.0, vy=0.0, vz=0.0, profile=None, profile_real=None):
         self.d.inflow_vx, self.d.inflow_vy, self.d.inflow_vz = vx, vy, vz
         self._profiles = []
+        f32 = profile_real == "f32"
+        self.d.inflow_profile_real = _abi.REAL_F32 if f32 else _abi.REAL_F64
         for k in range(3):
             if profile is not None and profile[k] is not None:
-                p = np.ascontiguousarray(profile[k], dtype=np.float64)  # indexed Y*z + y: shape (Z, Y)
+                # indexed Y*z + y: shape (Z, Y)
+                p = np.ascontiguousarray(profile[k], dtype=np.float32 if f32 else np.float64)
                 self._profiles.append(p)
                 self.d.inflow_profile[k] = p.ctypes.data
             else:

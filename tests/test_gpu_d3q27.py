@@ -113,6 +113,27 @@ def test_channel_bcs_match_oracle(pattern, inflow, outflow):
             assert rel_max(a.cpu().numpy(), b) < 5e-13  # small transverse velocities: FMA rounding
 
 
+@pytest.mark.parametrize("pattern", ["AA", "AB"])
+def test_f32_channel_with_float_inflow_profiles_matches_oracle(pattern):
+    """TraitsSP's NSE_Data_InflowProfile holds float arrays (lbm_data.h:100-104):
+    inflow_profile_real = F32 reads them as such; all three components vary."""
+    X, Y, Z = 12, 9, 8
+    geo = channel_geo(X, Y, Z, GEO.INFLOW, GEO.OUTFLOW_EQ)
+    g, h = pair(X, Y, Z, 0, storage="f32", pattern=pattern, geo=geo, nu=0.02)
+    rng = np.random.default_rng(5)
+    prof = [(0.02 + 0.005 * rng.standard_normal((Z, Y))).astype(np.float32),
+            (0.002 * rng.standard_normal((Z, Y))).astype(np.float32),
+            (0.002 * rng.standard_normal((Z, Y))).astype(np.float32)]
+    for lat in (g, h):
+        lat.set_equilibrium(1.0, 0.02, 0.0, 0.0)
+        lat.set_inflow(0.0, 0.0, 0.0, profile=prof, profile_real="f32")
+    assert g.d.inflow_profile_real == _abi.REAL_F32
+    for t in range(6):
+        g.step(output_macro=True)
+        h.step(output_macro=True)
+        assert cmp_pops(g, h) < 2e-6, t
+
+
 @pytest.mark.parametrize("pattern,inflow,outflow", [("AA", GEO.INFLOW, GEO.OUTFLOW_EQ),
                                                      ("AB", GEO.INFLOW_EQ_LEFT, GEO.OUTFLOW_RIGHT)])
 @pytest.mark.parametrize("storage,tol", [("f32dev", 2e-5), ("f16dev", 3e-3), ("bf16dev", 2e-2)])

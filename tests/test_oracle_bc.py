@@ -225,3 +225,28 @@ def test_d3q7_outflow_right_pulls_the_column_behind(coll):
     # uncollided pull of column x - 1 differs from the stored result)
     pull = np.array([[[gb[q, xo - CX[q], (y - CY[q]) % Y, (z - CZ[q]) % Z] for z in range(Z)] for y in range(Y)] for q in range(7)])
     assert np.abs(outs[0][:, xo] - pull).max() > 1e-6
+
+
+def test_inflow_site_holds_the_profile_equilibrium_for_float_and_double_profiles():
+    """An INFLOW site takes rho = 1 and u = inflow(y, z), then f = f_eq
+    (bc.h:86-90, NSE_Data_InflowProfile::inflow, lbm_data.h:104-132): after
+    one step each inflow site holds the product equilibrium of its profile
+    value (the default cumulant leaves it in place).  A float profile (TraitsSP's dreal, inflow_profile_real = F32) is
+    read as float: the same values in a double profile give the same bits."""
+    geo = np.full((X, Y, Z), GEO.PERIODIC, dtype=np.int16)
+    geo[3] = GEO.INFLOW
+    rng = np.random.default_rng(9)
+    prof32 = [(0.02 * rng.standard_normal((Z, Y))).astype(np.float32) for _ in range(3)]
+    out = {}
+    for kind in ("f32", "f64"):
+        h, _ = lattice(geo, seed=3)
+        prof = prof32 if kind == "f32" else [p.astype(np.float64) for p in prof32]
+        h.set_inflow(0.5, 0.5, 0.5, profile=prof, profile_real=kind)
+        h.step()
+        out[kind] = h.export_f64()
+    assert np.array_equal(out["f32"], out["f64"])
+    f = out["f32"]
+    for y in range(Y):
+        for z in range(Z):
+            u = [float(prof32[k][z, y]) for k in range(3)]
+            assert np.abs(f[:, 3, y, z] - O.equilibrium(1.0, *u, kind=0)).max() < 1e-15  # collided: a fixed point

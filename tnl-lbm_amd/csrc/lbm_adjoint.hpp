@@ -399,9 +399,9 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 						mu[q] = 0;
 					if (! SPECIAL && P.grad) {
 						const int64_t iy = A.Y * z + y;
-						const real ux = (real) (A.prof[0] ? A.prof[0][iy] : A.inflow[0]);
-						const real uy = (real) (A.prof[1] ? A.prof[1][iy] : A.inflow[1]);
-						const real uz = (real) (A.prof[2] ? A.prof[2][iy] : A.inflow[2]);
+						const real ux = (real) inflow_component(A, 0, iy);
+						const real uy = (real) inflow_component(A, 1, iy);
+						const real uz = (real) inflow_component(A, 2, iy);
 						real gx = 0, gy = 0, gz = 0;
 						D::eq_grad(lam, ux, uy, uz, gx, gy, gz);
 						const int64_t n = A.Y * A.Z;

@@ -256,8 +256,11 @@ static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 	A.nu = SD->lbm_viscosity;
 	A.fx = SD->fx; A.fy = SD->fy; A.fz = SD->fz;
 	A.inflow[0] = SD->inflow_vx; A.inflow[1] = SD->inflow_vy; A.inflow[2] = SD->inflow_vz;
+	if (SD->inflow_profile_real != TNL_LBM_REAL_F64 && SD->inflow_profile_real != TNL_LBM_REAL_F32)
+		return fail(TNL_LBM_ERR_ARG, "unknown inflow profile element type");
 	for (int k = 0; k < 3; k++)
 		A.prof[k] = SD->inflow_profile[k];
+	A.prof_f32 = SD->inflow_profile_real == TNL_LBM_REAL_F32;
 	grid = padded;
 	return TNL_LBM_OK;
 }

@@ -49,11 +49,22 @@ struct StepArgs
 	double nu;  // lbmViscosity (MRT_LES and KBC use it directly)
 	double fx, fy, fz;
 	double inflow[3];
-	const double* prof[3];
+	const void* prof[3];  // inflow profiles: double arrays, or float where prof_f32
+	int32_t prof_f32;
 	void* tape;  // moment recording of the A-B step (10 planes of XYZ), or null
 };
 
 constexpr int NXCD = 8;
+
+// component k of the inflow velocity at profile index i = Y z + y
+// (NSE_Data_InflowProfile::vel_*, lbm_data.h:104-125; the constant of
+// NSE_Data_ConstInflow where no profile is given)
+LBM_DEV double inflow_component(const StepArgs& A, int k, int64_t i)
+{
+	if (! A.prof[k])
+		return A.inflow[k];
+	return A.prof_f32 ? (double) static_cast<const float*>(A.prof[k])[i] : static_cast<const double*>(A.prof[k])[i];
+}
 
 // logical block -> (x, z, y segment) with XCD-aware remapping
 // LIN = false: whole rows only (x and z stay uniform per block, as the
@@ -696,9 +707,9 @@ struct General
 	{
 		// NSE_Data_ConstInflow / NSE_Data_InflowProfile::inflow (lbm_data.h:85-132)
 		const int64_t i = A.Y * z + y;
-		vx = (real) (A.prof[0] ? A.prof[0][i] : A.inflow[0]);
-		vy = (real) (A.prof[1] ? A.prof[1][i] : A.inflow[1]);
-		vz = (real) (A.prof[2] ? A.prof[2][i] : A.inflow[2]);
+		vx = (real) inflow_component(A, 0, i);
+		vy = (real) inflow_component(A, 1, i);
+		vz = (real) inflow_component(A, 2, i);
 	}
 
 	static LBM_DEV void site(const StepArgs& A, int64_t x, int64_t y, int64_t z)

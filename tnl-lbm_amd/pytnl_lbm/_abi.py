@@ -122,7 +122,7 @@ class D3Q27Data(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("version", C.c_uint32),
                 ("X", C.c_int64), ("Y", C.c_int64), ("Z", C.c_int64),
                 ("ox", C.c_int64), ("oy", C.c_int64), ("oz", C.c_int64), ("XYZ", C.c_int64),
-                ("even_iter", C.c_int32), ("reserved0", C.c_int32),
+                ("even_iter", C.c_int32), ("inflow_profile_real", C.c_int32),
                 ("lbm_viscosity", C.c_double), ("fx", C.c_double), ("fy", C.c_double), ("fz", C.c_double),
                 ("df", C.c_void_p * 2), ("macro", C.c_void_p), ("map", C.c_void_p),
                 ("inflow_vx", C.c_double), ("inflow_vy", C.c_double), ("inflow_vz", C.c_double),

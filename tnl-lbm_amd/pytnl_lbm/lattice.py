@@ -121,13 +121,20 @@ class D3Q27Lattice:
         if hasattr(self, "iterations"):
             self._bind()
 
-    def set_inflow(self, vx=0.0, vy=0.0, vz=0.0, profile=None):
-        """NSE_Data_ConstInflow values, or NSE_Data_InflowProfile arrays of shape (Z, Y)."""
+    def set_inflow(self, vx=0.0, vy=0.0, vz=0.0, profile=None, profile_real=None):
+        """NSE_Data_ConstInflow values, or NSE_Data_InflowProfile arrays of shape (Z, Y).
+        profile_real: element type of the profile arrays, "f64" (default) or
+        "f32" (the reference's TraitsSP profiles hold float, lbm_data.h:100-104)."""
         self.d.inflow_vx, self.d.inflow_vy, self.d.inflow_vz = vx, vy, vz
         self._profiles = [None, None, None]
+        f32 = profile_real == "f32"
+        if profile_real not in (None, "f64", "f32"):
+            raise ValueError(f"profile_real must be 'f64' or 'f32', not {profile_real!r}")
+        self.d.inflow_profile_real = _abi.REAL_F32 if f32 else _abi.REAL_F64
         for k in range(3):
             if profile is not None and profile[k] is not None:
-                p = torch.as_tensor(profile[k], dtype=torch.float64, device=self.device).contiguous()
+                p = torch.as_tensor(profile[k], dtype=torch.float32 if f32 else torch.float64,
+                                    device=self.device).contiguous()
                 assert p.numel() == self.Y * self.Z
                 self._profiles[k] = p
                 self.d.inflow_profile[k] = p.data_ptr()
