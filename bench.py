@@ -50,7 +50,7 @@ def parse():
     p.add_argument("--adjoint-steps", type=int, default=2000)
     p.add_argument("--adjoint-fd-dirs", type=int, default=16)
     p.add_argument("--coupled-steps", type=int, default=1000)
-    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--cpu-quick", action="store_true", help="CPU baseline at small sizes (a check of the leg)")
     p.add_argument("--storage", default="f64", choices=["f64", "f32", "f32dev", "f16dev", "bf16dev", "i16dev"],
                    help="storage form of the headline case (f64 is the headline workload; the others exist "
                         "so that each kernel can be profiled through this script)")
@@ -400,32 +400,34 @@ def coupled_adjoint_case(N, T, storage, layers=32, fd_dirs=0, grad_out=False):
     return (res, grad) if grad_out else res
 
 
-def cpu_baseline(seconds):
-    """The oracle's FP64 A-B pull step (the reference default pattern,
-    defs.h:20-26), OpenMP collapse(2) over x,z as in state.hpp:1388-1397."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
-    import oracle as O
-    from pytnl_lbm import _abi
-    from pytnl_lbm.fields import turbulence_fields
+def cpu_baseline(quick=False):
+    """The oracle's FP64 host step (a restatement of the reference's
+    Host/OpenMP path: LBMKernel in the loop of state.hpp:1388-1397) on every
+    CPU this job is given, in a child process that starts with
+    OMP_PROC_BIND=spread and OMP_PLACES=cores: the 64^3 Taylor-Green vortex
+    for 1000 steps and a 256^3 turbulence box for 20 steps, A-B and A-A, best
+    of 3 (oracle/cpu_baseline.py).  The value is the 256^3 A-A sample."""
+    import subprocess
 
-    threads = min(16, os.cpu_count() or 1)
-    N = 128
-    rho, vx, vy, vz = turbulence_fields(N, 1, 8.0 * N / 768, 0.04)
-    h = O.HostLattice(N, N, N, pattern=_abi.PATTERN_AB, nu=(0.5015 - 0.5) / 3.0, threads=threads)
-    h.set_equilibrium_field(rho, vx, vy, vz)
-    h.step()
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        h.step()
-        n += 1
-        dt = time.perf_counter() - t0
-        if dt > seconds and n >= 2:
-            break
-    return {"value": N ** 3 * n / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "sample": f"oracle FP64 D3Q27 cumulant A-B pull, {N}^3 periodic turbulence, {n} steps, "
-                      f"{threads} OpenMP threads"}
+    cpus = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env_threads = os.environ.get("OMP_NUM_THREADS")
+    threads = min(cpus, int(env_threads)) if env_threads and env_threads.isdigit() else cpus
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+    cmd = [sys.executable, str(ROOT / "oracle" / "cpu_baseline.py"), "--threads", str(threads)]
+    if quick:
+        cmd.append("--quick")
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, check=True)
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    head = [s for s in res["samples"] if s["pattern"] == "AA"][-1]
+    host = res["host"]
+    return {"value": head["mlups"], "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle FP64 D3Q27 cumulant (restatement of the reference Host/OpenMP path), "
+                      f"{head['sample']}, A-A, best of {head['repeats']}, {threads} OpenMP threads "
+                      f"(OMP_PROC_BIND=spread, OMP_PLACES=cores)",
+            "host": {"model": host.get("model"), "sockets": host.get("sockets"),
+                     "physical_cores": host.get("physical_cores"), "nproc": host.get("nproc"),
+                     "cpus_given_to_this_job": cpus},
+            "samples": res["samples"]}
 
 
 def calibrate_copy(nbytes=2 << 30, reps=5):
@@ -554,7 +556,8 @@ def main():
         out["adjoint"] = adj
         progress("coupled adjoint 192^3 done")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_quick)
+        progress(f"cpu baseline: {out['cpu_baseline']['value']} MLUPS on {out['cpu_baseline']['cores']} threads")
     print(json.dumps(out), flush=True)
 
 

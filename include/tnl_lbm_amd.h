@@ -288,6 +288,18 @@ int tnl_lbm_d3q7_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* 
 int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
                                double out[6], void* stream);
 
+/* ---- FP64 velocity-gradient reductions -----------------------------------
+ * The enstrophy diagnostic of sim_4.cu:282-300 (StateLocal::probe1): with the
+ * velocity gradient G_ab = (v_a(r + e_b) - v_a(r - e_b)) / 2 of
+ * computeGradVel (sim_4.cu:20-60), in lattice units, from the macro array:
+ *   out[0] = sum rho |G|^2        (the reference divides by rho_0 2 V)
+ *   out[1] = sum rho |curl v|^2   (computeVorticity, sim_4.cu:62-82)
+ * Neighbours wrap periodically on an axis that is not distributed
+ * (flags TNL_LBM_FLAG_DISTRIBUTED_*) and come from the overlap layer on one
+ * that is.  `out` is a HOST pointer; the call synchronises `stream`.       */
+int tnl_lbm_d3q27_reduce_gradients(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, uint32_t flags,
+                                   double out[2], void* stream);
+
 /* ---- halo exchange of an x-slab split ------------------------------------
  * One lattice split into x-slabs, each stored with a ghost plane per side
  * (ox >= 1) and stepped with TNL_LBM_FLAG_DISTRIBUTED_X.  `lower`'s +x face

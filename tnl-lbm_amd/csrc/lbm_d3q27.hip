@@ -517,32 +517,92 @@ int tnl_lbm_d3q27_halo_x(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* lo
 	return TNL_LBM_OK;
 }
 
-int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, double out[6], void* stream)
+// Partial sums of the FP64 reductions: one buffer per (host thread,
+// device), allocated on that thread's first reduction on the device and held
+// until the process ends (RED_BLOCKS + 1 rows of RED_N doubles, 49 KB).
+// Each reduction synchronises its stream before it returns, so the next one
+// of the same thread may reuse the buffer on any stream.
+static int partial_buffer(double** out)
 {
-	if (int rc = validate(cfg, SD))
-		return rc;
-	if (! SD->macro || ! out)
-		return fail(TNL_LBM_ERR_NULL, "macro array or output missing");
-	// per-device partial-sum buffer, allocated once
-	static thread_local int dev_cached = -1;
-	static thread_local double* partial = nullptr;
+	constexpr int MAX_DEVICES = 64;
+	static thread_local double* bufs[MAX_DEVICES] = {};
 	int dev = 0;
 	if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice"))
 		return rc;
-	if (dev != dev_cached) {
-		if (int rc = hip_check(hipMalloc(&partial, sizeof(double) * (RED_BLOCKS * RED_N + RED_N)), "hipMalloc reduce buffer"))
+	if (dev < 0 || dev >= MAX_DEVICES)
+		return fail(TNL_LBM_ERR_ARG, "device ordinal beyond the reduction buffers' table");
+	if (! bufs[dev])
+		if (int rc = hip_check(hipMalloc(&bufs[dev], sizeof(double) * (RED_BLOCKS * RED_N + RED_N)), "hipMalloc reduce buffer"))
 			return rc;
-		dev_cached = dev;
-	}
-	const MacroArgs M = macro_args(cfg, SD, SD->df[0]);
+	*out = bufs[dev];
+	return TNL_LBM_OK;
+}
+
+// partial sums -> out (device) on `st`
+static int finish_reduction(double* partial, double* out_dev, hipStream_t st)
+{
+	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*) partial, RED_BLOCKS, out_dev);
+	return hip_check(hipGetLastError(), "reduce launch");
+}
+
+static int reduction_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, MacroArgs& M)
+{
+	if (int rc = validate(cfg, SD))
+		return rc;
+	if (! SD->macro)
+		return fail(TNL_LBM_ERR_NULL, "macro array missing");
+	M = macro_args(cfg, SD, SD->df[0]);
+	M.dist_x = M.dist_y = M.dist_z = 0;
+	return TNL_LBM_OK;
+}
+
+int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, double out[6], void* stream)
+{
+	MacroArgs M;
+	if (int rc = reduction_args(cfg, SD, M))
+		return rc;
+	if (! out)
+		return fail(TNL_LBM_ERR_NULL, "output missing");
+	double* partial = nullptr;
+	if (int rc = partial_buffer(&partial))
+		return rc;
 	hipStream_t st = (hipStream_t) stream;
 	hipLaunchKernelGGL(k_reduce_macro, dim3(RED_BLOCKS), dim3(256), 0, st, M, partial);
-	hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, st, (const double*) partial, RED_BLOCKS, partial + RED_BLOCKS * RED_N);
-	if (int rc = hip_check(hipGetLastError(), "reduce launch"))
+	if (int rc = finish_reduction(partial, partial + RED_BLOCKS * RED_N, st))
 		return rc;
 	if (int rc = hip_check(hipMemcpyAsync(out, partial + RED_BLOCKS * RED_N, sizeof(double) * RED_N, hipMemcpyDeviceToHost, st), "reduce copy"))
 		return rc;
 	return hip_check(hipStreamSynchronize(st), "reduce sync");
+}
+
+int tnl_lbm_d3q27_reduce_gradients(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, uint32_t flags, double out[2],
+								   void* stream)
+{
+	MacroArgs M;
+	if (int rc = reduction_args(cfg, SD, M))
+		return rc;
+	if (! out)
+		return fail(TNL_LBM_ERR_NULL, "output missing");
+	M.dist_x = (flags & TNL_LBM_FLAG_DISTRIBUTED_X) != 0;
+	M.dist_y = (flags & TNL_LBM_FLAG_DISTRIBUTED_Y) != 0;
+	M.dist_z = (flags & TNL_LBM_FLAG_DISTRIBUTED_Z) != 0;
+	if ((M.dist_x && SD->ox < 1) || (M.dist_y && SD->oy < 1) || (M.dist_z && SD->oz < 1))
+		return fail(TNL_LBM_ERR_DIMS, "a distributed axis needs an overlap of at least one site");
+	double* partial = nullptr;
+	if (int rc = partial_buffer(&partial))
+		return rc;
+	hipStream_t st = (hipStream_t) stream;
+	hipLaunchKernelGGL(k_reduce_gradients, dim3(RED_BLOCKS), dim3(256), 0, st, M, partial);
+	if (int rc = finish_reduction(partial, partial + RED_BLOCKS * RED_N, st))
+		return rc;
+	double res[RED_N];
+	if (int rc = hip_check(hipMemcpyAsync(res, partial + RED_BLOCKS * RED_N, sizeof(double) * RED_N, hipMemcpyDeviceToHost, st), "reduce copy"))
+		return rc;
+	if (int rc = hip_check(hipStreamSynchronize(st), "reduce sync"))
+		return rc;
+	out[0] = res[0];
+	out[1] = res[1];
+	return TNL_LBM_OK;
 }
 
 }  // extern "C"

@@ -1208,6 +1208,7 @@ struct MacroArgs
 	int macro_f64;
 	int64_t XYZ;
 	int64_t X, Y, Z, ox, oy, oz, Ys, Zs;
+	int32_t dist_x, dist_y, dist_z;  // gradient reduction: neighbours from the overlaps, not wrapped
 };
 
 // LBM_BLOCK::computeInitialMacro (lbm_block.hpp:281-305); forces zeroed
@@ -1299,6 +1300,52 @@ static __global__ void __launch_bounds__(256) k_reduce_macro(const MacroArgs A, 
 		v[3] += (rho - 1.0) * (rho - 1.0);
 		v[4] += u2;
 		v[5] += 1.0;
+	}
+	block_sum(v, sh);
+	if (threadIdx.x == 0)
+#pragma unroll
+		for (int i = 0; i < RED_N; i++)
+			partial[blockIdx.x * RED_N + i] = sh[0][i];
+}
+
+// Velocity-gradient reductions of sim_4.cu:282-300: with the central
+// differences G_ab = (v_a(r + e_b) - v_a(r - e_b)) / 2 of computeGradVel
+// (sim_4.cu:20-60; periodic wrap on an axis that is not distributed, the
+// overlap layer on one that is), per block:
+//   [0] sum rho |G|^2       (the enstrophy integrand of sim_4.cu:284-298)
+//   [1] sum rho |curl v|^2  (computeVorticity, sim_4.cu:62-82)
+// entries 2..5 of each partial row are zero.
+static __global__ void __launch_bounds__(256) k_reduce_gradients(const MacroArgs A, double* partial)
+{
+	__shared__ double sh[256][RED_N];
+	double v[RED_N] = {0, 0, 0, 0, 0, 0};
+	const int64_t N = A.X * A.Y * A.Z;
+	auto at = [&](int id, int64_t x, int64_t y, int64_t z) -> double {
+		const int64_t s = (int64_t) id * A.XYZ + ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys + (y + A.oy);
+		return A.macro_f64 ? ((const double*) A.macro)[s] : (double) ((const float*) A.macro)[s];
+	};
+	for (int64_t n = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (int64_t) gridDim.x * blockDim.x) {
+		const int64_t y = n % A.Y, t = n / A.Y, z = t % A.Z, x = t / A.Z;
+		const int64_t xp = (! A.dist_x && x == A.X - 1) ? 0 : x + 1, xm = (! A.dist_x && x == 0) ? A.X - 1 : x - 1;
+		const int64_t yp = (! A.dist_y && y == A.Y - 1) ? 0 : y + 1, ym = (! A.dist_y && y == 0) ? A.Y - 1 : y - 1;
+		const int64_t zp = (! A.dist_z && z == A.Z - 1) ? 0 : z + 1, zm = (! A.dist_z && z == 0) ? A.Z - 1 : z - 1;
+		double G[3][3];  // G[a][b] = d v_a / d x_b
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+			G[a][0] = 0.5 * (at(1 + a, xp, y, z) - at(1 + a, xm, y, z));
+			G[a][1] = 0.5 * (at(1 + a, x, yp, z) - at(1 + a, x, ym, z));
+			G[a][2] = 0.5 * (at(1 + a, x, y, zp) - at(1 + a, x, y, zm));
+		}
+		const double rho = at(0, x, y, z);
+		double g2 = 0;
+#pragma unroll
+		for (int a = 0; a < 3; a++)
+#pragma unroll
+			for (int b = 0; b < 3; b++)
+				g2 += G[a][b] * G[a][b];
+		const double wx = G[2][1] - G[1][2], wy = G[0][2] - G[2][0], wz = G[1][0] - G[0][1];
+		v[0] += rho * g2;
+		v[1] += rho * (wx * wx + wy * wy + wz * wz);
 	}
 	block_sum(v, sh);
 	if (threadIdx.x == 0)

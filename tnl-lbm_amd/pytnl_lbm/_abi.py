@@ -230,6 +230,9 @@ def load_library():
     lib.tnl_lbm_d3q27_reduce_macro.argtypes = [C.POINTER(Config), C.POINTER(D3Q27Data),
                                                C.POINTER(C.c_double), _P]
     lib.tnl_lbm_d3q27_reduce_macro.restype = C.c_int
+    lib.tnl_lbm_d3q27_reduce_gradients.argtypes = [C.POINTER(Config), C.POINTER(D3Q27Data), C.c_uint32,
+                                                   C.POINTER(C.c_double), _P]
+    lib.tnl_lbm_d3q27_reduce_gradients.restype = C.c_int
     if lib.tnl_lbm_abi_version() != ABI_VERSION:
         raise RuntimeError("libtnl_lbm_amd.so ABI version mismatch")
     _lib = lib

@@ -265,6 +265,15 @@ class D3Q27Lattice:
         self._check(self.lib.tnl_lbm_d3q27_reduce_macro(C.byref(self.cfg), C.byref(self.d), out, _stream_ptr()))
         return list(out)
 
+    def reduce_gradients(self, flags=0):
+        """FP64 sums of rho |grad u|^2 and rho |curl u|^2 over the macro array with
+        the central differences of sim_4.cu's computeGradVel (sim_4.cu:20-82):
+        the enstrophy diagnostic before its 1/(2 rho_0 V) normalisation."""
+        out = (C.c_double * 2)()
+        self._check(self.lib.tnl_lbm_d3q27_reduce_gradients(C.byref(self.cfg), C.byref(self.d), flags, out,
+                                                            _stream_ptr()))
+        return list(out)
+
 
 class D3Q7Lattice:
     """The advection-diffusion block of State_NSE_ADE (state_NSE_ADE.h:3-401):
