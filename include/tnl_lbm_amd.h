@@ -288,6 +288,20 @@ int tnl_lbm_d3q7_export_f64(const tnl_lbm_config* cfg, const tnl_lbm_d3q7_data* 
 int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
                                double out[6], void* stream);
 
+/* ---- FP64 reductions without allocation ------------------------------------
+ * tnl_lbm_d3q27_reduce_macro with the same kernels and the same six results,
+ * written to `out_device` (a DEVICE array of 6 doubles) asynchronously on
+ * `stream`: no allocation, no copy to the host, no synchronisation.  No
+ * counterpart in the reference: it is for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Its partial sums live in a fixed device scratch of 1024 rows of 6 doubles
+ * (48 KiB per GPU, held until the process ends); the reduction grid never
+ * needs more, whatever the lattice size.  All calls on one GPU share that
+ * scratch: calls that overlap on one GPU (two streams, two host threads)
+ * race and must be ordered by the caller.                                   */
+int tnl_lbm_d3q27_reduce_macro_noalloc(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, double* out_device,
+                                       void* stream);
+
 /* ---- FP64 velocity-gradient reductions -----------------------------------
  * The enstrophy diagnostic of sim_4.cu:282-300 (StateLocal::probe1): with the
  * velocity gradient G_ab = (v_a(r + e_b) - v_a(r - e_b)) / 2 of

@@ -17,6 +17,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -163,6 +164,38 @@ MacroArgs macro_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, co
 }
 
 }  // namespace
+
+// Scratch of the allocation-free reduction: the partial sums of its
+// RED_BLOCKS workgroups (48 KiB), one copy per GPU, held until the process
+// ends.  Host code never passes `scratch` itself (that name is a host-side
+// stand-in); it passes the device address hipGetSymbolAddress returns, kept
+// per device in g_scratch_addr.
+constexpr int SCRATCH_ROWS = RED_BLOCKS;
+static __device__ __align__(256) double scratch[SCRATCH_ROWS * RED_N];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_addr[SCRATCH_MAX_DEVICES] = {};
+
+// this device's address of `scratch`, looked up on the first call per device
+static int scratch_address(double** out)
+{
+	int dev = 0;
+	if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice"))
+		return rc;
+	if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+		return fail(TNL_LBM_ERR_ARG, "device ordinal beyond the scratch table");
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if (! g_scratch_addr[dev]) {
+		void* p = nullptr;
+		if (int rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)"))
+			return rc;
+		if (! p)
+			return fail(TNL_LBM_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+		g_scratch_addr[dev] = static_cast<double*>(p);
+	}
+	*out = g_scratch_addr[dev];
+	return TNL_LBM_OK;
+}
 
 // ===========================================================================
 extern "C" {
@@ -573,6 +606,28 @@ int tnl_lbm_d3q27_reduce_macro(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_da
 	if (int rc = hip_check(hipMemcpyAsync(out, partial + RED_BLOCKS * RED_N, sizeof(double) * RED_N, hipMemcpyDeviceToHost, st), "reduce copy"))
 		return rc;
 	return hip_check(hipStreamSynchronize(st), "reduce sync");
+}
+
+int tnl_lbm_d3q27_reduce_macro_noalloc(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, double* out_device,
+									   void* stream)
+{
+	MacroArgs M;
+	if (int rc = reduction_args(cfg, SD, M))
+		return rc;
+	if (! out_device)
+		return fail(TNL_LBM_ERR_NULL, "device output missing");
+	// every one of the RED_BLOCKS workgroups writes its own partial row, so
+	// nothing in the scratch needs clearing; the grid never exceeds the rows
+	static_assert(RED_BLOCKS <= SCRATCH_ROWS, "the reduction grid must fit the scratch");
+	const int blocks = RED_BLOCKS;
+	if (blocks > SCRATCH_ROWS)
+		return fail(TNL_LBM_ERR_DIMS, "reduction grid larger than the scratch (1024 partial rows)");
+	double* partial = nullptr;
+	if (int rc = scratch_address(&partial))
+		return rc;
+	hipStream_t st = (hipStream_t) stream;
+	hipLaunchKernelGGL(k_reduce_macro, dim3(blocks), dim3(256), 0, st, M, partial);
+	return finish_reduction(partial, out_device, st);
 }
 
 int tnl_lbm_d3q27_reduce_gradients(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, uint32_t flags, double out[2],
