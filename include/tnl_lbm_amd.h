@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define TNL_LBM_ABI_VERSION 2u
+#define TNL_LBM_ABI_VERSION 3u
 
 /* ---- status codes ------------------------------------------------------ */
 #define TNL_LBM_OK 0
@@ -195,6 +195,16 @@ typedef struct tnl_lbm_d3q27_data {
 	 * deviations differ by orders of magnitude (the rest population's is
 	 * ~1.5|u|^2), so per-class bounds buy up to 6 bits of resolution.       */
 	double store_range_class[4];
+	/* MACRO::computeForcing (d3q27/macro.h:50-52, called at kernels.h:103 and
+	 * :162): a per-site force added to fx, fy, fz before the collision -- three
+	 * planes (fx, fy, fz) of XYZ elements of cfg->macro_real in the lattice
+	 * layout, or NULL for none.  This is the form in which the reference's
+	 * IBM (the macro quantities e_fx..e_fz, lagrange_3D.hpp) and
+	 * non-Newtonian (nonNewtonian.h:679-760) force terms reach the kernel.
+	 * The map kernel and the coupled kernel read it; a box without a map and
+	 * with a force field is stepped by the map kernel as all PERIODIC.  The
+	 * adjoint entry points do not take it (TNL_LBM_ERR_UNSUPPORTED).         */
+	const void* force_field;
 } tnl_lbm_d3q27_data;
 
 /* D3Q7 ADE data: ADE_Data (lbm_data.h:212-246) plus a source term that the

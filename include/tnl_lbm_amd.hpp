@@ -258,6 +258,19 @@ inline tnl_lbm_d3q7_data ade_data_for(DATA& D, double phi_ref = 0.0, double stor
 	return ad;
 }
 
+// The per-site force of a MACRO with force quantities e_fx, e_fy, e_fz (the
+// IBM forcing of D3Q27_MACRO_Base, d3q27/macro.h:15-25, and the
+// non-Newtonian MACRO, nonNewtonian.h:575-625): those macro planes are the
+// three planes tnl_lbm_d3q27_data::force_field expects, so set
+//     sd.force_field = tnl_lbm_amd::macro_force_field<NSE::MACRO>(block.data);
+template <typename MACRO, typename DATA>
+inline const void* macro_force_field(DATA& D)
+{
+	static_assert(MACRO::e_fy == MACRO::e_fx + 1 && MACRO::e_fz == MACRO::e_fx + 2,
+				  "the force quantities must be consecutive macro planes");
+	return D.dmacro + (long) MACRO::e_fx * D.XYZ;
+}
+
 // flags of the reference's call: distributed (x, y, z) and output_macro
 template <typename DIST>
 inline uint32_t flags_for(const DIST& distributed, bool output_macro)

@@ -90,6 +90,28 @@ double inflow_at(const tnl_lbm_d3q27_data* SD, int k, int64_t i)
 	return static_cast<const double*>(SD->inflow_profile[k])[i];
 }
 
+// MACRO::computeForcing (macro.h:50-52, kernels.h:103): the per-site force
+// field, three planes of macro_real elements, added to the homogeneous force
+// that copyQuantities put into the kernel struct (macro.h:80-86)
+template <typename real>
+void add_site_force(const tnl_lbm_d3q27_data* SD, int mreal, int64_t s, real& fx, real& fy, real& fz)
+{
+	if (! SD->force_field)
+		return;
+	if (mreal == TNL_LBM_REAL_F64) {
+		const double* F = static_cast<const double*>(SD->force_field);
+		fx += (real) F[s];
+		fy += (real) F[SD->XYZ + s];
+		fz += (real) F[2 * SD->XYZ + s];
+	}
+	else {
+		const float* F = static_cast<const float*>(SD->force_field);
+		fx += (real) F[s];
+		fy += (real) F[SD->XYZ + s];
+		fz += (real) F[2 * SD->XYZ + s];
+	}
+}
+
 // rest weights w_q = W(cx) W(cy) W(cz), W(0)=2/3, W(+-1)=1/6
 double w27(int q)
 {
@@ -1311,6 +1333,7 @@ struct Site27
 		KS.fx = (real) SD->fx;
 		KS.fy = (real) SD->fy;
 		KS.fz = (real) SD->fz;
+		add_site_force(SD, mreal, G.s(x, y, z), KS.fx, KS.fy, KS.fz);
 		pre_collision(G, SD, KS, gi_map, xm, x, xp, ym, y, yp, zm, z, zp);
 		if (do_collision(gi_map))
 			collide(KS);
@@ -1769,6 +1792,7 @@ int run_coupled(const tnl_lbm_d3q27_data* SD, const tnl_lbm_d3q7_data* ADE, cons
 				KS.fx = (typename N::real) SD->fx;
 				KS.fy = (typename N::real) SD->fy;
 				KS.fz = (typename N::real) SD->fz;
+				add_site_force(SD, mreal, s, KS.fx, KS.fy, KS.fz);
 				N::pre_collision(G, SD, KS, gi, xm, x, xp, ym, y, yp, zm, z, zp);
 				if (N::do_collision(gi))
 					N::collide(KS);
@@ -2346,6 +2370,8 @@ int oracle_d3q27_adjoint_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_dat
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the absolute storage forms");
 	if (SD->ox || SD->oy || SD->oz)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for a single block without overlaps");
+	if (SD->force_field)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the homogeneous force only");
 	if (! lambda_next || ! lambda)
 		return fail(TNL_LBM_ERR_NULL, "adjoint arrays missing");
 	if (cfg->storage == TNL_LBM_STORE_F64)

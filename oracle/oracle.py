@@ -143,6 +143,21 @@ class HostLattice:
     def from_storage(self, v):
         return v.reshape(self.X, self.Z, self.Y).transpose(0, 2, 1)
 
+    def set_force_field(self, fx=None, fy=None, fz=None):
+        """The per-site force of MACRO::computeForcing: [x, y, z] arrays or None."""
+        if fx is None and fy is None and fz is None:
+            self._force = None
+            self.d.force_field = None
+            return
+        dt = np.float64 if self.cfg.macro_real == _abi.REAL_F64 else np.float32
+        t = np.zeros((3, self.d.XYZ), dtype=dt)
+        n = self.X * self.Y * self.Z
+        for k, a in enumerate((fx, fy, fz)):
+            if a is not None:
+                t[k, :n] = self.to_storage(a).astype(dt)
+        self._force = t
+        self.d.force_field = t.ctypes.data
+
     def set_inflow(self, vx=0.0, vy=0.0, vz=0.0, profile=None, profile_real=None):
         self.d.inflow_vx, self.d.inflow_vy, self.d.inflow_vz = vx, vy, vz
         self._profiles = []

@@ -85,3 +85,18 @@ def test_halo_validation_without_gpu():
     lo.df[1] = up.df[1] = 0x2000
     assert lib.tnl_lbm_d3q27_halo_x(C.byref(ab), C.byref(lo), C.byref(up), _abi.HALO_PUSH, None) == _abi.ERR_UNSUPPORTED
     assert lib.tnl_lbm_d3q27_halo_x(None, C.byref(lo), C.byref(up), _abi.HALO_PULL, None) == _abi.ERR_NULL
+
+
+def test_adjoint_rejects_a_force_field_without_gpu():
+    """The adjoint entry points differentiate the homogeneous-force step only:
+    a per-site force field is refused before any launch."""
+    lib = _abi.load_library()
+    cfg = _abi.Config.make(pattern=_abi.PATTERN_AB)
+    d = _abi.D3Q27Data.make(4, 4, 4)
+    d.df[0] = d.df[1] = 0x1000
+    d.lbm_viscosity = 0.01
+    d.force_field = 0x3000
+    assert lib.tnl_lbm_d3q27_adjoint_step(C.byref(cfg), C.byref(d), C.c_void_p(0x4000), C.c_void_p(0x5000), None,
+                                          None) == _abi.ERR_UNSUPPORTED
+    assert b"force_field" in lib.tnl_lbm_last_error()
+    assert lib.tnl_lbm_d3q27_step_record(C.byref(cfg), C.byref(d), C.c_void_p(0x6000), 0, None) == _abi.ERR_UNSUPPORTED

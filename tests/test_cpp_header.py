@@ -16,6 +16,8 @@ from pathlib import Path
 
 import pytest
 
+from pytnl_lbm import _abi
+
 ROOT = Path(__file__).resolve().parents[1]
 LIBDIR = ROOT / "tnl-lbm_amd" / "pytnl_lbm"
 
@@ -87,6 +89,11 @@ struct AdePlainSP : Base<float> {
     AdePlainSP() { dfs[0] = s1; if (DFMAX > 1) dfs[DFMAX - 1] = s0; dmacro = sm; }
 };
 
+// a MACRO with the IBM force quantities (d3q27/macro.h:15-25)
+struct MacroIBM { enum : unsigned char { N, e_rho, e_vx, e_vy, e_vz, e_fx, e_fy, e_fz }; };
+static double big[8 * 777];
+struct WithForce : Base<double> { WithForce() { dfs[0] = f0; dmacro = big; } };
+
 template <typename N, typename DATA>
 int unused_launch(DATA& d) { return tnl_lbm_amd::step<N>(d, Idx3{0, 0, 0}, Idx3{5, 6, 7}, Dist{false, false, false}, true); }
 template <typename N, typename A, typename D1, typename D2>
@@ -123,6 +130,8 @@ int main()
                 ad.map == amp, ad.diffusion_coefficient == sd, ad.source == nullptr, ad.inflow_phi, ad.phi_ref,
                 ad.store_scale, ad2.collision, ad2.inflow_phi, ad2.diffusion_coefficient == nullptr);
     std::printf("flags %u %u\n", flags_for(Dist{true, false, true}, true), flags_for(Dist{false, true, false}, false));
+    WithForce wf;
+    std::printf("force %d\n", macro_force_field<MacroIBM>(wf) == (const void*) (big + 5 * 777));
     if (false) {
         unused_launch<NSE<TraitsDP, D3Q27_CUM<TraitsDP>>>(a);
         unused_coupled<NSE<TraitsSP, D3Q27_CUM<TraitsSP>>, NSE<TraitsSP, D3Q7_SRT<TraitsSP>>>(c, e);
@@ -157,7 +166,7 @@ def test_header_maps_reference_types(tmp_path, defines, cum, pattern):
     assert cfg[1:5] == [str(cum), "0", str(pattern), "0"]
     assert cfg[6:9] == ["3", "1", "1"]
     assert cfg[10] == "7" and cfg[12:14] == ["8", "5"]
-    assert cfg[16] == "2"  # ABI version
+    assert cfg[16] == str(_abi.ABI_VERSION)  # ABI version
     aa = "AA_PATTERN" in defines
     data = out[1].split()[1:]
     assert data[:7] == ["5", "6", "7", "1", "2", "3", "777"]
@@ -173,6 +182,7 @@ def test_header_maps_reference_types(tmp_path, defines, cum, pattern):
     assert ade[16:19] == ["0.75", "0.25", "1000"]
     assert ade[20:23] == ["1", "1", "1"]  # D3Q7_MRT; no inflow_phi member -> 1; no per-site D
     assert out[5].split()[1:] == [str(1 | 2 | 8), str(4)]
+    assert out[6].split()[1:] == ["1"]  # the force planes e_fx..e_fz of the macro array
     assert aa == (cfg[3] == "0")
 
 

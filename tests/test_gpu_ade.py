@@ -140,3 +140,19 @@ def test_coupled_reduced_storage_matches_oracle(storage, tol):
         da = ga.export_f64().cpu().numpy() - W7
         db = ha.export_f64() - W7
         assert rel_max(da, db) < tol, (storage, t, rel_max(da, db))
+
+
+@pytest.mark.parametrize("pattern", ["AA", "AB"])
+def test_coupled_step_with_site_force_field(pattern):
+    """computeForcing in the coupled kernel (kernels.h:162): the NSE site
+    reads the per-site force; the scalar is advected by the forced flow."""
+    X, Y, Z = 8, 10, 6
+    g, h, ga, ha = pair(X, Y, Z, "CLBM", pattern=pattern)
+    init(g, h, ga, ha, 31)
+    F = 1e-4 * np.random.default_rng(32).standard_normal((3, X, Y, Z))
+    g.set_force_field(*F)
+    h.set_force_field(*F)
+    for t in range(6):
+        g.step_coupled(ga, output_macro=True)
+        h.step_coupled(ha, output_macro=True)
+        compare(g, h, ga, ha, 1e-13, 1e-13)

@@ -482,3 +482,50 @@ def test_tgv_64_fp64_parity_1000_steps(coll):
     worst = run_tgv(64, 1000, 100, coll)
     print(f"TGV 64^3 coll={coll}: worst max-rel error {worst:.3e}")
     assert worst <= 1e-10
+
+
+@pytest.mark.parametrize("pattern,coll,storage,tol", [("AA", 0, "f64", 1e-13), ("AB", 0, "f64", 1e-13),
+                                                      ("AA", 7, "f64", 1e-13), ("AA", 3, "f64", 1e-13),
+                                                      ("AA", 0, "f32", 2e-6), ("AB", 0, "f32", 2e-6)])
+def test_site_force_field_channel_matches_oracle(pattern, coll, storage, tol):
+    """MACRO::computeForcing's per-site force (force_field) on a channel with
+    walls, inflow and outflow, added to a homogeneous force, in the map
+    kernel: populations and macros against the oracle every step."""
+    X, Y, Z = 12, 9, 8
+    geo = channel_geo(X, Y, Z, GEO.INFLOW, GEO.OUTFLOW_EQ if pattern == "AA" else GEO.OUTFLOW_RIGHT)
+    g, h = pair(X, Y, Z, coll, storage=storage, pattern=pattern, geo=geo, nu=0.02, force=(1e-5, 0.0, -2e-5))
+    rng = np.random.default_rng(17)
+    F = 1e-4 * rng.standard_normal((3, X, Y, Z))
+    for lat in (g, h):
+        lat.set_equilibrium(1.0, 0.02, 0.0, 0.0)
+        lat.set_inflow(0.02, 0.001, 0.0)
+        lat.set_force_field(*F)
+    for t in range(6):
+        g.step(output_macro=True)
+        h.step(output_macro=True)
+        assert cmp_pops(g, h) < tol, (t, cmp_pops(g, h))
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 2e-6), ("f16dev", 3e-3)])
+def test_site_force_field_without_map_runs_the_map_kernel(storage, tol):
+    """A box without a map but with a force field: stepped as all PERIODIC by
+    the map kernel (the map-free kernels take the homogeneous force only)."""
+    X, Y, Z = 6, 70, 5
+    g, h = pair(X, Y, Z, 0, storage=storage)
+    r = fields(X, Y, Z, 8, amp=0.02)
+    g.set_equilibrium_field(*r)
+    h.set_equilibrium_field(*r)
+    F = 1e-4 * np.random.default_rng(9).standard_normal((3, X, Y, Z))
+    g.set_force_field(*F)
+    h.set_force_field(*F)
+    Wb = W[:, None, None, None] if storage == "f16dev" else 0.0
+    for t in range(5):
+        g.step(output_macro=True)
+        h.step(output_macro=True)
+        assert rel_max(g.export_f64().cpu().numpy() - Wb, h.export_f64() - Wb) < tol, t
+    # the field really acts: without it the populations differ
+    g2, _ = pair(X, Y, Z, 0, storage=storage)
+    g2.set_equilibrium_field(*r)
+    for t in range(5):
+        g2.step()
+    assert rel_max(g2.export_f64().cpu().numpy() - Wb, h.export_f64() - Wb) > 10 * tol

@@ -57,6 +57,9 @@ MUTANTS = {
     "cum2017_eq83_index": ("Cs_122 + (ks_020 * ks_102 + ks_002 * ks_120 + no4 * ks_011 * ks_111",
                            "Cs_122 + (ks_020 * ks_102 + ks_002 * ks_120 + no4 * ks_101 * ks_111",
                            "test_oracle_cumulant_space.py", "relaxation_rules"),
+    "site_force_plane": ("		fz += (real) F[2 * SD->XYZ + s];\n	}\n	else {",
+                         "		fz += (real) F[SD->XYZ + s];\n	}\n	else {",
+                         "test_oracle_forcing.py", "own_force"),
     "cum_default_cs110_squared": ("const dreal ks_220 = (ks_020 * ks_200 + no2 * ks_110 * ks_110) * rho_inv;",
                                   "const dreal ks_220 = (ks_020 * ks_200 + ks_110 * ks_110) * rho_inv;",
                                   "test_oracle_cumulant_space.py", "relaxation_rules"),

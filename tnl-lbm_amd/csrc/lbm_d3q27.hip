@@ -294,6 +294,7 @@ static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 	for (int k = 0; k < 3; k++)
 		A.prof[k] = SD->inflow_profile[k];
 	A.prof_f32 = SD->inflow_profile_real == TNL_LBM_REAL_F32;
+	A.force = SD->force_field;
 	grid = padded;
 	return TNL_LBM_OK;
 }
@@ -673,6 +674,8 @@ static int adjoint_config(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the absolute storage forms (F64, F32)");
 	if (SD->ox || SD->oy || SD->oz)
 		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for a single block without overlaps");
+	if (SD->force_field)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "the adjoint is built for the homogeneous force only (no force_field)");
 	return TNL_LBM_OK;
 }
 

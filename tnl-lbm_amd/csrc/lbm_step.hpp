@@ -51,10 +51,31 @@ struct StepArgs
 	double inflow[3];
 	const void* prof[3];  // inflow profiles: double arrays, or float where prof_f32
 	int32_t prof_f32;
+	const void* force;  // per-site force: 3 planes of macro elements (macro_f64), or null
 	void* tape;  // moment recording of the A-B step (10 planes of XYZ), or null
 };
 
 constexpr int NXCD = 8;
+
+// per-site force of MACRO::computeForcing (kernels.h:103): added to the
+// homogeneous force of copyQuantities (macro.h:80-86) in the arithmetic of
+// the collision
+template <typename real>
+LBM_DEV void add_site_force(const StepArgs& A, int64_t s, real& Fx, real& Fy, real& Fz)
+{
+	if (A.macro_f64) {
+		const double* F = (const double*) A.force;
+		Fx += (real) F[s];
+		Fy += (real) F[A.XYZ + s];
+		Fz += (real) F[2 * A.XYZ + s];
+	}
+	else {
+		const float* F = (const float*) A.force;
+		Fx += (real) F[s];
+		Fy += (real) F[A.XYZ + s];
+		Fz += (real) F[2 * A.XYZ + s];
+	}
+}
 
 // component k of the inflow velocity at profile index i = Y z + y
 // (NSE_Data_InflowProfile::vel_*, lbm_data.h:104-125; the constant of
@@ -123,7 +144,8 @@ LBM_DEV void write_macro(const StepArgs& A, int64_t s, real rho, real vx, real v
 // COLL::collision after computeDensityAndVelocity, in the reference's operation order
 template <int ST, int COLL>
 LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real rho, typename Storage<ST>::real vx,
-					 typename Storage<ST>::real vy, typename Storage<ST>::real vz, const StepArgs& A)
+					 typename Storage<ST>::real vy, typename Storage<ST>::real vz, const StepArgs& A,
+					 typename Storage<ST>::real Fx, typename Storage<ST>::real Fy, typename Storage<ST>::real Fz)
 {
 	using P = Phys<ST>;
 	using real = typename Storage<ST>::real;
@@ -142,8 +164,7 @@ LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real r
 		// KBC_N1..N4 = 4..7, KBC_C1..C4 = 8..11: shear part D, D+T, D+Q, D+T+Q
 		constexpr int k = (COLL - TNL_LBM_COLL_KBC_N1) % 4;
 		constexpr bool central = COLL >= TNL_LBM_COLL_KBC_C1;
-		P::template kbc<central, k == 1 || k == 3, k >= 2>(f, rho, vx, vy, vz, (real) A.nu, (real) A.fx, (real) A.fy,
-														   (real) A.fz);
+		P::template kbc<central, k == 1 || k == 3, k >= 2>(f, rho, vx, vy, vz, (real) A.nu, Fx, Fy, Fz);
 	}
 }
 
@@ -185,7 +206,7 @@ LBM_DEV void collide_site(typename Storage<ST>::real* f, const StepArgs& A, type
 	}
 	else {
 		Phys<ST>::density_velocity(f, (real) A.fx, (real) A.fy, (real) A.fz, rho, vx, vy, vz);
-		collide<ST, COLL>(f, rho, vx, vy, vz, A);
+		collide<ST, COLL>(f, rho, vx, vy, vz, A, (real) A.fx, (real) A.fy, (real) A.fz);
 	}
 }
 
@@ -730,7 +751,9 @@ struct General
 		NX = neighbours<AA>(x, A.X, A.ox, A.dist_x, per);
 		NY = neighbours<AA>(y, A.Y, A.oy, A.dist_y, per);
 		NZ = neighbours<AA>(z, A.Z, A.oz, A.dist_z, per);
-		const real Fx = (real) A.fx, Fy = (real) A.fy, Fz = (real) A.fz;
+		real Fx = (real) A.fx, Fy = (real) A.fy, Fz = (real) A.fz;
+		if (A.force)
+			add_site_force(A, s0, Fx, Fy, Fz);
 		real f[27];
 		real rho = 1, vx = 0, vy = 0, vz = 0;
 		const elem* __restrict__ cur = (const elem*) A.df0;
@@ -918,10 +941,10 @@ struct General
 						f[q] *= FC::unscale(q);
 				}
 				else if (do_coll)
-					collide<ST, COLL>(f, rho, vx, vy, vz, A);
+					collide<ST, COLL>(f, rho, vx, vy, vz, A, Fx, Fy, Fz);
 			}
 			else if (do_coll)
-				collide<ST, COLL>(f, rho, vx, vy, vz, A);
+				collide<ST, COLL>(f, rho, vx, vy, vz, A, Fx, Fy, Fz);
 			// postCollision streaming
 			if (AA && A.even) {
 				elem* __restrict__ out = (elem*) A.df0;
@@ -1410,7 +1433,8 @@ inline bool pair_shape(const StepArgs& A, StepArgs& B, unsigned& threads, unsign
 template <int ST, int COLL>
 int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
 {
-	if (! A.map && cfg->pattern == TNL_LBM_PATTERN_AA) {
+	// the map-free periodic kernels take the homogeneous force only
+	if (! A.map && ! A.force && cfg->pattern == TNL_LBM_PATTERN_AA) {
 		if constexpr (sizeof(typename Storage<ST>::elem) == 2 && ((LBM_PAIR_EVEN >> ST) & 1)) {
 			StepArgs B;
 			unsigned t2 = 0, g2 = 0;

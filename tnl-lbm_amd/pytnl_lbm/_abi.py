@@ -10,7 +10,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # status codes
 OK = 0
@@ -127,7 +127,7 @@ class D3Q27Data(C.Structure):
                 ("df", C.c_void_p * 2), ("macro", C.c_void_p), ("map", C.c_void_p),
                 ("inflow_vx", C.c_double), ("inflow_vy", C.c_double), ("inflow_vz", C.c_double),
                 ("inflow_profile", C.c_void_p * 3), ("store_range", C.c_double),
-                ("store_range_class", C.c_double * 4)]
+                ("store_range_class", C.c_double * 4), ("force_field", C.c_void_p)]
 
     @classmethod
     def make(cls, X, Y, Z, ox=0, oy=0, oz=0, XYZ=None):

@@ -106,6 +106,7 @@ class D3Q27Lattice:
         self.d.df[1] = self.df[1].data_ptr() if len(self.df) > 1 else None
         self.d.macro = self.macro.data_ptr()
         self.d.map = self.map.data_ptr() if self.map is not None else None
+        self.d.force_field = self.force_field.data_ptr() if getattr(self, "force_field", None) is not None else None
 
     def _check(self, rc):
         _abi.check(self.lib, rc)
@@ -120,6 +121,26 @@ class D3Q27Lattice:
             self.map = to_storage(g).reshape(-1).to(self.device)
         if hasattr(self, "iterations"):
             self._bind()
+
+    def set_force_field(self, fx=None, fy=None, fz=None):
+        """MACRO::computeForcing's per-site force (macro.h:50-52, called at
+        kernels.h:103), added to the homogeneous force before each collision:
+        [x, y, z] arrays over the stored lattice (overlaps included), None for
+        a zero component; all three None removes the field.  Kept as three
+        planes of the macro element type in the lattice layout."""
+        if fx is None and fy is None and fz is None:
+            self.force_field = None
+        else:
+            mdt = torch.float64 if self.cfg.macro_real == _abi.REAL_F64 else torch.float32
+            Xs, Ys, Zs = self.X + 2 * self.ox, self.Y + 2 * self.oy, self.Z + 2 * self.oz
+            t = torch.zeros(3, self.d.XYZ, dtype=mdt, device=self.device)
+            for k, a in enumerate((fx, fy, fz)):
+                if a is not None:
+                    a = torch.as_tensor(a, dtype=mdt)
+                    assert tuple(a.shape) == (Xs, Ys, Zs), (tuple(a.shape), (Xs, Ys, Zs))
+                    t[k, :Xs * Ys * Zs] = to_storage(a).reshape(-1).to(self.device)
+            self.force_field = t
+        self._bind()
 
     def set_inflow(self, vx=0.0, vy=0.0, vz=0.0, profile=None, profile_real=None):
         """NSE_Data_ConstInflow values, or NSE_Data_InflowProfile arrays of shape (Z, Y).
@@ -190,6 +211,10 @@ class D3Q27Lattice:
 
     def current_df(self):
         return self.iterations % 2 if self.cfg.pattern == _abi.PATTERN_AB else 0
+
+    def synchronize(self):
+        """Wait for the work queued on this lattice's device."""
+        torch.cuda.synchronize(self.device)
 
     def halo_x(self, upper: "D3Q27Lattice", phase: int, stream=None):
         """Halo exchange of an x-slab split (tnl_lbm_d3q27_halo_x): this slab's
