@@ -23,7 +23,8 @@ CQ = 0.05
 
 
 def fake_lattice():
-    cfg = types.SimpleNamespace(pattern=_abi.PATTERN_AB, collision=_abi.COLL_CUM, storage=_abi.STORE_F64)
+    cfg = types.SimpleNamespace(pattern=_abi.PATTERN_AB, collision=_abi.COLL_CUM, storage=_abi.STORE_F64,
+                                macro_real=_abi.REAL_F64)
     d = types.SimpleNamespace(inflow_profile=[None, None, None], XYZ=N // 27)
     return types.SimpleNamespace(cfg=cfg, d=d, df=[torch.zeros(N, dtype=torch.float64)], device=torch.device("cpu"),
                                  X=1, Y=Y, Z=Z, _profiles=[None] * 3, _bind=lambda: None,
@@ -39,23 +40,29 @@ def quad(f):
 class FakeRun(CheckpointedRun):
     def __init__(self, T, K, A, B, mode, budget=None):
         self.A, self.B, self.nsteps, self.nadj = A, B, 0, 0
-        super().__init__(fake_lattice(), T, K, mode=mode, tape_f64=True, record_budget=budget)
+        super().__init__(fake_lattice(), T, K, mode=mode, tape_f64=mode != "mixed", record_budget=budget)
 
-    def _step(self, src, dst, tape=None, stream=None):
+    def _refresh_fp32(self):
+        pass
+
+    def _step(self, src, dst, tape=None, stream=None, fp32=False):
         assert src.data_ptr() != dst.data_ptr()
+        assert fp32 == (src.dtype == torch.float32) == (dst.dtype == torch.float32)
+        A, Bp = self.A.to(src.dtype), (self.B @ self.profile.reshape(-1)).to(src.dtype)
         if tape is not None:
             tape.copy_(src[:W])
-        dst.copy_(self.A @ src + self.B @ self.profile.reshape(-1) + quad(src))
+        dst.copy_(A @ src + Bp + quad(src))
         self.nsteps += 1
 
-    def _adjoint(self, state, lam_next, lam, grad, tape=None, stream=None):
+    def _adjoint(self, state, lam_next, lam, grad, tape=None, stream=None, fp32=False):
         assert lam_next.data_ptr() != lam.data_ptr()
+        assert fp32 == (lam.dtype == torch.float32)
         part = state[:W] if tape is None else tape
         assert (state is None) == (tape is not None)
-        out = self.A.T @ lam_next
-        out[:W] += 2 * CQ * part * lam_next[:W]
+        out = self.A.to(lam.dtype).T @ lam_next
+        out[:W] += 2 * CQ * part.to(lam.dtype) * lam_next[:W]
         lam.copy_(out)
-        grad += (self.B.T @ lam_next).reshape(grad.shape)
+        grad += (self.B.T @ lam_next.double()).reshape(grad.shape)
         self.nadj += 1
 
 
@@ -63,9 +70,12 @@ STEP_BYTES = W * 8  # one step's float64 recording of the fake lattice
 
 
 @pytest.mark.parametrize("mode,budget", [("recompute", None), ("segment", None), ("tape", None),
-                                         ("hybrid", 0), ("hybrid", 12 * STEP_BYTES), ("hybrid", 1000 * STEP_BYTES)])
+                                         ("hybrid", 0), ("hybrid", 12 * STEP_BYTES), ("hybrid", 1000 * STEP_BYTES),
+                                         ("mixed", None)])
 @pytest.mark.parametrize("T,K", [(1, None), (7, 1), (7, 7), (16, None), (17, None), (23, 5)])
 def test_gradient_equals_unrolled_autograd(mode, budget, T, K):
+    if mode == "mixed" and T > 20:
+        pytest.skip("this map's gradient reaches 1e43 after 23 steps, past float32's range")
     g = torch.Generator().manual_seed(T * 31 + (K or 0))
     # a contraction, so that 23 steps of the quadratic map stay bounded
     A = 0.7 * torch.eye(N, dtype=torch.float64) + 0.2 / N**0.5 * torch.randn(N, N, generator=g, dtype=torch.float64)
@@ -95,7 +105,7 @@ def test_gradient_equals_unrolled_autograd(mode, budget, T, K):
         assert recomputed == 0 and run.tape.shape[0] == T and not run.ckpt
     elif mode == "recompute":
         assert recomputed == T - run.nseg and len(run.seg) == run.K - 1
-    elif mode == "segment":  # every step of every segment is recorded once more
+    elif mode in ("segment", "mixed"):  # every step of every segment is recorded once more
         assert recomputed == T and run.tape.shape[0] == run.K
     else:  # the recorded trailing segments are not recomputed, the others are, with recording
         assert recomputed == T - run.recorded_steps
@@ -104,8 +114,13 @@ def test_gradient_equals_unrolled_autograd(mode, budget, T, K):
             assert run.recorded_steps == T
         if budget == 0:
             assert run.recorded_steps == 0
-    np.testing.assert_allclose(pc.grad.numpy(), pr.grad.numpy(), rtol=1e-11, atol=1e-11 * pr.grad.abs().max().item())
-    np.testing.assert_allclose(fc.grad.numpy(), fr.grad.numpy(), rtol=1e-11, atol=1e-11 * fr.grad.abs().max().item())
+    # the mixed schedule recomputes and differentiates in float32: float rounding
+    tol = 1e-4 if mode == "mixed" else 1e-11
+    np.testing.assert_allclose(pc.grad.numpy(), pr.grad.numpy(), rtol=tol, atol=tol * pr.grad.abs().max().item())
+    np.testing.assert_allclose(fc.grad.numpy(), fr.grad.numpy(), rtol=tol, atol=tol * fr.grad.abs().max().item())
+    if mode == "mixed":  # but not further off than that: float rounding of a few steps, not a wrong state
+        err = (pc.grad - pr.grad).norm() / pr.grad.norm()
+        assert err < 1e-5, err
 
 
 def test_rejects_a_to_a_pattern_and_unknown_modes():

@@ -132,7 +132,7 @@ def test_flow_step_reproduces_the_coupled_flow():
     assert torch.equal(g.macro[n:], coupled_m[n:])
 
 
-def scalar_problem(storage, N=(12, 10, 24), T=40, layers=6, K=None, seed=2):
+def scalar_problem(storage, N=(12, 10, 24), T=40, layers=6, K=None, seed=2, host=False):
     from pytnl_lbm import CoupledScalarRun, scalar_run, top_layer_variance
     X, Y, Z = N
     geo, ageo = geometry("zwalls", X, Y, Z)
@@ -145,7 +145,7 @@ def scalar_problem(storage, N=(12, 10, 24), T=40, layers=6, K=None, seed=2):
     g.set_equilibrium_field(np.ones((X, Y, Z)), *u)
     a.set_equilibrium(0.0)
     f0, g0 = g.df[0].clone(), a.df[0].clone()
-    run = CoupledScalarRun(g, a, T, K)
+    run = CoupledScalarRun(g, a, T, K, host_checkpoints=host)
     S = torch.tensor(1e-3 * rng.random(g.d.XYZ), device="cuda")
 
     def J(s):
@@ -183,6 +183,20 @@ def test_source_gradient_independent_of_the_stride():
         grads.append(Sr.grad.cpu().numpy())
     for gr in grads[1:]:
         assert np.abs(gr - grads[0]).max() <= 1e-13 * np.abs(grads[0]).max()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16dev"])
+def test_host_checkpoints_give_the_same_gradient(storage):
+    """Flow checkpoints in host memory: the same states, so the same gradient
+    bit for bit (K = 7 puts 6 checkpoints and a ragged last segment in play)."""
+    grads = []
+    for host in (False, True):
+        _, run, S, J = scalar_problem(storage, K=7, host=host)
+        assert run.ckpt[0].device.type == ("cpu" if host else "cuda")
+        Sr = S.clone().requires_grad_()
+        J(Sr).backward()
+        grads.append(Sr.grad.cpu().numpy())
+    assert np.array_equal(grads[0], grads[1])
 
 
 @pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16dev", 5e-2)])

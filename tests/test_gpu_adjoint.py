@@ -107,14 +107,18 @@ def duct_problem(X=32, Y=16, Z=16, T=120, xp=24, storage="f64", K=None, mode="re
     return lat, run, p, f0, J
 
 
-@pytest.mark.parametrize("mode,tape_f64", [("recompute", None), ("segment", None), ("tape", False), ("hybrid", False)])
+@pytest.mark.parametrize("mode,tape_f64", [("recompute", None), ("segment", None), ("tape", False), ("hybrid", False),
+                                           ("mixed", None)])
 def test_inflow_gradient_matches_central_differences(mode, tape_f64):
     """The duct with a perturbed inlet profile, objective 1/2 sum |u - u_target|^2
     on one x plane after T steps: dJ/d(profile) . d against
     (J(p + eps d) - J(p - eps d)) / (2 eps), eps = 1e-4, on 6 random directions,
     within 1e-5 relative; and dJ/d(initial state) on 2 directions.  For each
-    schedule: states recomputed, moments recorded per segment (float64), and
-    moments recorded for the whole run (float32)."""
+    schedule: states recomputed, moments recorded per segment (float64),
+    moments recorded for the whole run (float32), and the mixed schedule (FP64
+    forward and checkpoints, FP32 recomputation and adjoint), whose FP32
+    backward pass is held to the same bound against the FP64 run's
+    differences."""
     lat, run, p, f0, J = duct_problem(mode=mode, tape_f64=tape_f64)
     pr, fr = p.clone().requires_grad_(), f0.clone().requires_grad_()
     Jv = J(pr, fr)
@@ -154,6 +158,87 @@ def test_gradient_does_not_depend_on_the_schedule():
     for key, g in grads.items():
         tol = 1e-6 if key == ("tape", None, False) else 1e-13
         assert rel_max(g, ref) < tol, (key, rel_max(g, ref))
+
+
+def test_mixed_schedule_close_to_the_fp64_schedules():
+    """The FP32 recomputation starts from an FP64 checkpoint every K steps, so
+    its gradient departs from the FP64 one by FP32 rounding over at most K
+    steps (and the FP32 adjoint's own rounding): ~1e-6 relative."""
+    out = {}
+    for mode in ("segment", "mixed"):
+        _, _, p, f0, J = duct_problem(T=120, mode=mode)
+        pr = p.clone().requires_grad_()
+        J(pr, f0).backward()
+        out[mode] = pr.grad.cpu().numpy()
+    err = np.linalg.norm(out["mixed"] - out["segment"]) / np.linalg.norm(out["segment"])
+    assert err < 1e-5, err
+
+
+def test_backward_refuses_checkpoints_of_a_later_forward():
+    """One CheckpointedRun holds the checkpoints of its latest forward pass: a
+    graph whose forward was followed by another forward cannot be
+    differentiated with them."""
+    _, run, p, f0, J = duct_problem(T=16)
+    pr = p.clone().requires_grad_()
+    J1 = J(pr, f0)
+    with torch.no_grad():
+        J(p, f0)
+    with pytest.raises(RuntimeError, match="forward again"):
+        J1.backward()
+
+
+def test_flat_inflow_gets_a_flat_gradient():
+    from pytnl_lbm import lattice_run
+    lat, run, p, f0, _ = duct_problem(T=8)
+    pr = p.reshape(-1).clone().requires_grad_()
+    lattice_run(run, pr, f0).sum().backward()
+    assert pr.grad.shape == pr.shape
+
+
+def test_ab_adjoint_differentiates_the_aa_run():
+    """A-A steps from S f0 (S: pure streaming) give the A-B trajectory from
+    f0, step by step; a uniform initial state is its own stream (S f0 = f0),
+    so from it the A-A and A-B runs coincide and the A-B adjoint's gradient is
+    the gradient of the A-A run: dJ/d(profile) of the A-B run against central
+    differences of an A-A run of the same duct, J from the A-A step's macro
+    output at step T (the pre-collision rho, u of that step, which the
+    collision conserves).  INFLOW in, OUTFLOW_EQ out: the zero-gradient
+    OUTFLOW_RIGHT reads the column behind it after streaming and exists for
+    the pull scheme only."""
+    from pytnl_lbm import CheckpointedRun, channel_geometry, density_velocity, inflow_control, lattice_run, populations
+    X, Y, Z, T, xp = 32, 16, 16, 60, 24
+    geo = channel_geometry(X, Y, Z, outflow=GEO.OUTFLOW_EQ)
+    nu = (0.51 - 0.5) / 3
+    ab = D3Q27Lattice(X, Y, Z, pattern="AB", nu=nu, geo=geo)
+    ab.set_equilibrium(1.0, 0.02, 0.0, 0.0)
+    f0 = ab.df[0].clone()
+    run = CheckpointedRun(ab, T)
+    p = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=1), device="cuda")
+    ut = torch.full((3, Z, Y), 0.01, dtype=torch.float64, device="cuda")
+    pr = p.clone().requires_grad_()
+    _, u = density_velocity(populations(lattice_run(run, pr, f0), ab)[:, xp])
+    (0.5 * ((u - ut) ** 2).sum()).backward()
+
+    def J_aa(pp):
+        aa = D3Q27Lattice(X, Y, Z, pattern="AA", nu=nu, geo=geo)
+        aa.set_equilibrium(1.0, 0.02, 0.0, 0.0)
+        aa.set_inflow(0.0, 0.0, 0.0, profile=[pp[0], pp[1], pp[2]])
+        for _ in range(T - 1):
+            aa.step()
+        aa.step(output_macro=True)  # T is even: the A-A populations are in place again
+        rho, ux, uy, uz = aa.macros()
+        u_aa = torch.stack([ux[xp], uy[xp], uz[xp]]).permute(0, 2, 1)
+        return 0.5 * ((u_aa - ut) ** 2).sum().item()
+
+    eps = 1e-4
+    gen = torch.Generator(device="cuda").manual_seed(8)
+    for _ in range(3):
+        d = torch.randn(p.shape, generator=gen, device="cuda", dtype=torch.float64)
+        d[:, 0, :] = d[:, -1, :] = 0
+        d[:, :, 0] = d[:, :, -1] = 0
+        fd = (J_aa(p + eps * d) - J_aa(p - eps * d)) / (2 * eps)
+        ad = (pr.grad * d).sum().item()
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (ad, fd)
 
 
 def test_f32_gradient_close_to_f64():

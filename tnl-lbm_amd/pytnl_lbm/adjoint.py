@@ -48,7 +48,7 @@ def density_velocity(f: torch.Tensor, force=(0.0, 0.0, 0.0)):
     return rho, (j + 0.5 * F) / rho
 
 
-MODES = ("recompute", "segment", "tape", "hybrid")
+MODES = ("recompute", "segment", "tape", "hybrid", "mixed")
 
 
 def tape_elements(XYZ: int) -> int:
@@ -86,6 +86,16 @@ class CheckpointedRun:
       once the other buffers exist); only the segments before those are
       recomputed backwards.  With enough memory nothing is recomputed; with
       none it is ``"segment"``.  Float recordings unless ``tape_f64``.
+    - ``"mixed"`` (F64 storage): the forward pass and its T/K checkpoints in
+      FP64; backwards, each segment's checkpoint is converted to FP32 and
+      recomputed by FP32 steps that record their moments (float), and the
+      adjoint runs in FP32 over that recording, accumulating the inflow
+      gradient in FP64.  Memory as ``"segment"``; per step the backward pass
+      moves 2 x (216 + 40) bytes against the forward's 432, so fwd+bwd runs at
+      about 0.45 of the forward rate.  The FP32 recomputation departs from the
+      FP64 trajectory by rounding only, over at most K steps from an FP64
+      state: the gradient agrees with central differences of the FP64 run to
+      ~1e-7 relative (tested).
 
     Recordings are float64 for F64 storage in ``"segment"`` mode (the
     gradient then equals the ``"recompute"`` one) and float32 otherwise,
@@ -105,10 +115,12 @@ class CheckpointedRun:
         if mode not in MODES:
             raise ValueError(f"mode must be one of {MODES}")
         f64 = lat.cfg.storage == _abi.STORE_F64
+        if mode == "mixed" and not f64:
+            raise ValueError("the mixed schedule runs an F64 forward pass: build the lattice with storage='f64'")
         if tape_f64 is None:
             tape_f64 = f64 and mode == "segment"
-        if tape_f64 and not f64:
-            raise ValueError("a float64 recording needs F64 storage")
+        if tape_f64 and (not f64 or mode == "mixed"):
+            raise ValueError("a float64 recording needs F64 storage and an F64 recomputation")
         self.lat, self.T, self.mode, self.tape_f64 = lat, T, mode, bool(tape_f64)
         self.K = K or max(1, math.ceil(math.sqrt(T)))
         self.nseg = math.ceil(T / self.K) if mode != "tape" else 1
@@ -121,9 +133,17 @@ class CheckpointedRun:
         self.ckpt = [z() for _ in range(self.nseg)] if mode != "tape" else []
         self.pp = [z() for _ in range(2)]
         self.seg = [z() for _ in range(self.K - 1)] if mode == "recompute" else []
-        ntape = {"recompute": 0, "segment": self.K, "tape": T, "hybrid": 0}[mode]
+        ntape = {"recompute": 0, "segment": self.K, "tape": T, "hybrid": 0, "mixed": self.K}[mode]
         self.tape = torch.zeros(ntape, tape_elements(lat.d.XYZ), dtype=tdt, device=dev) if ntape else None
-        self.lam = [z() for _ in range(2)]
+        if mode == "mixed":
+            # the FP32 lattice of the backward pass: same geometry, map, inflow and force
+            self.cfg32 = _abi.Config.make(lat.cfg.collision, _abi.STORE_F32, lat.cfg.pattern, lat.cfg.macro_real)
+            self.d32 = _abi.D3Q27Data()  # refreshed from the lattice's data at each backward pass
+            self.pp32 = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(2)]
+            self.lam = [torch.zeros(n, dtype=torch.float32, device=dev) for _ in range(2)]
+        else:
+            self.lam = [z() for _ in range(2)]
+        self.generation = 0  # forward passes so far: a backward belongs to the latest one
         self.rec_first, self.rec, self.rec_offset = self.nseg, None, {}
         if mode == "hybrid":
             self._plan_recording(tdt, record_budget)
@@ -162,34 +182,50 @@ class CheckpointedRun:
         self.rec = torch.zeros(total, el, dtype=tdt, device=lat.device) if total else None
         self.tape = torch.zeros(self.K, el, dtype=tdt, device=lat.device) if self.rec_first > 0 else None
 
+    def memory_bytes(self) -> int:
+        """Device memory the schedule holds: checkpoints, ping-pong states,
+        recordings and adjoint fields."""
+        bufs = list(self.ckpt) + list(self.pp) + list(self.seg) + list(self.lam) + list(getattr(self, "pp32", []))
+        for t in (self.tape, self.rec):
+            if t is not None:
+                bufs.append(t)
+        return sum(b.numel() * b.element_size() for b in bufs)
+
     @property
     def recorded_steps(self) -> int:
         """Steps whose moments the forward pass records (the others are recomputed)."""
         return 0 if self.rec is None else self.rec.shape[0]
 
     # ---- single launches ------------------------------------------------------
-    def _step(self, src: torch.Tensor, dst: torch.Tensor, tape: torch.Tensor | None = None, stream=None):
+    def _step(self, src: torch.Tensor, dst: torch.Tensor, tape: torch.Tensor | None = None, stream=None,
+              fp32: bool = False):
         lat = self.lat
-        lat.d.even_iter = 1
-        lat.d.df[0], lat.d.df[1] = src.data_ptr(), dst.data_ptr()
+        cfg, d = (self.cfg32, self.d32) if fp32 else (lat.cfg, lat.d)
+        d.even_iter = 1
+        d.df[0], d.df[1] = src.data_ptr(), dst.data_ptr()
         if tape is None:
-            lat._check(lat.lib.tnl_lbm_d3q27_step(C.byref(lat.cfg), C.byref(lat.d), _abi.I64x3(0, 0, 0),
+            lat._check(lat.lib.tnl_lbm_d3q27_step(C.byref(cfg), C.byref(d), _abi.I64x3(0, 0, 0),
                                                   _abi.I64x3(lat.X, lat.Y, lat.Z), 0, _stream_ptr(stream)))
         else:
-            lat._check(lat.lib.tnl_lbm_d3q27_step_record(C.byref(lat.cfg), C.byref(lat.d), _ptr(tape),
-                                                         int(self.tape_f64), _stream_ptr(stream)))
+            lat._check(lat.lib.tnl_lbm_d3q27_step_record(C.byref(cfg), C.byref(d), _ptr(tape), int(self.tape_f64),
+                                                         _stream_ptr(stream)))
 
     def _adjoint(self, state: torch.Tensor | None, lam_next: torch.Tensor, lam: torch.Tensor, grad: torch.Tensor,
-                 tape: torch.Tensor | None = None, stream=None):
+                 tape: torch.Tensor | None = None, stream=None, fp32: bool = False):
         lat = self.lat
+        cfg, d = (self.cfg32, self.d32) if fp32 else (lat.cfg, lat.d)
         if tape is None:
-            lat.d.df[0], lat.d.df[1] = state.data_ptr(), None
-            lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step(C.byref(lat.cfg), C.byref(lat.d), _ptr(lam_next),
-                                                          _ptr(lam), _ptr(grad), _stream_ptr(stream)))
+            d.df[0], d.df[1] = state.data_ptr(), None
+            lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step(C.byref(cfg), C.byref(d), _ptr(lam_next), _ptr(lam),
+                                                          _ptr(grad), _stream_ptr(stream)))
         else:
-            lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step_recorded(C.byref(lat.cfg), C.byref(lat.d), _ptr(tape),
+            lat._check(lat.lib.tnl_lbm_d3q27_adjoint_step_recorded(C.byref(cfg), C.byref(d), _ptr(tape),
                                                                    int(self.tape_f64), _ptr(lam_next), _ptr(lam),
                                                                    _ptr(grad), _stream_ptr(stream)))
+
+    def _refresh_fp32(self):
+        """The FP32 data struct takes the lattice's geometry, map, inflow profile and force."""
+        C.pointer(self.d32)[0] = self.lat.d
 
     def _other(self, src):
         return self.pp[1] if src is self.pp[0] else self.pp[0]
@@ -200,6 +236,7 @@ class CheckpointedRun:
         profile; returns the buffer holding the final state (valid until the
         next call)."""
         try:
+            self.generation += 1
             self.profile.copy_(inflow.reshape(3, self.lat.Z, self.lat.Y))
             if self.mode == "tape":
                 src = self.pp[0]
@@ -241,6 +278,17 @@ class CheckpointedRun:
                     for i in reversed(range(m)):
                         self._adjoint(None, cur, other, grad, self.rec[self.rec_offset[j] + i])
                         cur, other = other, cur
+                elif self.mode == "mixed":
+                    self._refresh_fp32()
+                    src = self.pp32[0]
+                    src.copy_(self.ckpt[j])  # FP64 checkpoint -> FP32 state
+                    for i in range(m):
+                        dst = self.pp32[1] if src is self.pp32[0] else self.pp32[0]
+                        self._step(src, dst, self.tape[i], fp32=True)
+                        src = dst
+                    for i in reversed(range(m)):
+                        self._adjoint(None, cur, other, grad, self.tape[i], fp32=True)
+                        cur, other = other, cur
                 elif self.mode == "recompute":
                     states = [self.ckpt[j]] + self.seg[:m - 1]
                     for i in range(m - 1):
@@ -267,14 +315,20 @@ class _LatticeRun(torch.autograd.Function):
     def forward(ctx, inflow, f0, run: CheckpointedRun):
         fT = run.forward(inflow.detach(), f0.detach())
         ctx.run = run
+        ctx.generation = run.generation
+        ctx.shapes = (inflow.shape, f0.shape, f0.dtype)
         return fT.clone()
 
     @staticmethod
     def backward(ctx, gF):
         run = ctx.run
+        if run.generation != ctx.generation:
+            raise RuntimeError("the CheckpointedRun was run forward again before this backward pass: its checkpoints "
+                               "belong to the later run (use one CheckpointedRun per graph)")
         grad = torch.zeros(3, run.lat.Z, run.lat.Y, dtype=torch.float64, device=gF.device)
         lam0 = run.backward(gF.to(run.lam[0].dtype).contiguous(), grad)
-        return grad, lam0.clone(), None
+        inflow_shape, f0_shape, f0_dtype = ctx.shapes
+        return grad.reshape(inflow_shape), lam0.to(f0_dtype).reshape(f0_shape), None
 
 
 def lattice_run(run: CheckpointedRun, inflow: torch.Tensor, f0: torch.Tensor) -> torch.Tensor:
@@ -298,11 +352,17 @@ class CoupledScalarRun:
     recomputed by plain D3Q27 steps that write their velocity (macro output
     into one buffer per step, the same velocity the coupled kernel hands to
     the D3Q7 update), and the D3Q7 adjoint steps (tnl_lbm_d3q7_adjoint_step)
-    run over it in reverse.  Memory: T/K flow states + K velocity fields.
-    Any storage form; the adjoint runs in the compute type (double for F64,
-    float otherwise) and accumulates the source gradient in double."""
+    run over it in reverse.  Memory: T/K flow states + K velocity fields,
+    packed in 3K + 1 planes: step i of a segment writes its macro output (rho,
+    vx, vy, vz) three planes below step i - 1's, so that only step i - 1's
+    unused density plane is overwritten.  ``host_checkpoints`` keeps the flow
+    checkpoints in host memory (copied at each checkpoint and at the start of
+    each segment's recomputation), for a lattice whose T/K device checkpoints
+    would not fit beside the K velocity fields (FP64 at 384^3: 12.2 GB per
+    state).  Any storage form; the adjoint runs in the compute type (double
+    for F64, float otherwise) and accumulates the source gradient in double."""
 
-    def __init__(self, nse: D3Q27Lattice, ade, T: int, K: int | None = None):
+    def __init__(self, nse: D3Q27Lattice, ade, T: int, K: int | None = None, host_checkpoints: bool = False):
         if nse.cfg.pattern != _abi.PATTERN_AB:
             raise ValueError("the adjoint differentiates the A-B step: build the lattices with pattern='AB'")
         if nse.cfg.collision != _abi.COLL_CUM:
@@ -318,10 +378,12 @@ class CoupledScalarRun:
         el = nse.df[0].dtype
         mdt = torch.float64 if nse.cfg.macro_real == _abi.REAL_F64 else torch.float32
         cdt = torch.float64 if nse.cfg.storage == _abi.STORE_F64 else torch.float32
-        self.ckpt = [torch.zeros(27 * n, dtype=el, device=dev) for _ in range(self.nseg)]
+        self.host = bool(host_checkpoints)
+        cdev = "cpu" if self.host else dev
+        self.ckpt = [torch.zeros(27 * n, dtype=el, device=cdev) for _ in range(self.nseg)]
         self.pp = [torch.zeros(27 * n, dtype=el, device=dev) for _ in range(2)]
         self.app = [torch.zeros(7 * n, dtype=el, device=dev) for _ in range(2)]
-        self.ubuf = [torch.zeros(4 * n, dtype=mdt, device=dev) for _ in range(self.K)]
+        self.vel = torch.zeros((3 * self.K + 1) * n, dtype=mdt, device=dev)
         self.lam = [torch.zeros(7 * n, dtype=cdt, device=dev) for _ in range(2)]
         self.out = torch.zeros(7 * n, dtype=torch.float64, device=dev)
         self.source = torch.zeros(n, dtype=mdt, device=dev)
@@ -348,6 +410,12 @@ class CoupledScalarRun:
         self._launch(nse.lib.tnl_lbm_d3q27_step(C.byref(nse.cfg), C.byref(nse.d), _abi.I64x3(0, 0, 0),
                                                 _abi.I64x3(nse.X, nse.Y, nse.Z), _abi.FLAG_OUTPUT_MACRO,
                                                 _stream_ptr(stream)))
+
+    def _macro_slot(self, i):
+        """The 4 planes step i of a segment writes its macro output to."""
+        n = self.nse.d.XYZ
+        base = 3 * (self.K - 1 - i) * n
+        return self.vel[base:base + 4 * n]
 
     def _adjoint(self, ubuf, lam_next, lam, grad, stream=None):
         nse = self.nse
@@ -377,12 +445,18 @@ class CoupledScalarRun:
             self.ckpt[0].copy_(f0.reshape(-1))
             self.app[0].copy_(g0.reshape(-1))
             fs, gs = self.ckpt[0], self.app[0]
+            if self.host:
+                self.pp[0].copy_(self.ckpt[0])
+                fs = self.pp[0]
             for t in range(self.T):
                 nxt = t + 1
-                fd = self.ckpt[nxt // self.K] if nxt % self.K == 0 and nxt < self.T else (
-                    self.pp[1] if fs is self.pp[0] else self.pp[0])
+                at_ckpt = nxt % self.K == 0 and nxt < self.T
+                other = self.pp[1] if fs is self.pp[0] else self.pp[0]
+                fd = self.ckpt[nxt // self.K] if at_ckpt and not self.host else other
                 gd = self.app[1] if gs is self.app[0] else self.app[0]
                 self._coupled(fs, fd, gs, gd)
+                if at_ckpt and self.host:
+                    self.ckpt[nxt // self.K].copy_(fd)
                 fs, gs = fd, gd
             return gs
         finally:
@@ -399,13 +473,17 @@ class CoupledScalarRun:
             for j in reversed(range(self.nseg)):
                 t0 = j * self.K
                 m = min(self.T, t0 + self.K) - t0
-                fs = self.ckpt[j]
+                if self.host:
+                    self.pp[0].copy_(self.ckpt[j])
+                    fs = self.pp[0]
+                else:
+                    fs = self.ckpt[j]
                 for i in range(m):
                     fd = self.pp[1] if fs is self.pp[0] else self.pp[0]
-                    self._flow_step(fs, fd, self.ubuf[i])
+                    self._flow_step(fs, fd, self._macro_slot(i))
                     fs = fd
                 for i in reversed(range(m)):
-                    self._adjoint(self.ubuf[i], cur, other, grad)
+                    self._adjoint(self._macro_slot(i), cur, other, grad)
                     cur, other = other, cur
             return cur
         finally:
