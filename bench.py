@@ -65,19 +65,20 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def timed_steps(lat, steps, warmup):
+def timed_steps(lat, steps, warmup, fn=None):
     """W untimed steps, then K steps bracketed by synchronisation; per-launch
     kernel durations from events recorded on the launch stream."""
     import torch
 
+    fn = fn or lat.step
     for _ in range(warmup):
-        lat.step()
+        fn()
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
     for i in range(steps):
         ev[i][0].record()
-        lat.step()
+        fn()
         ev[i][1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -85,7 +86,7 @@ def timed_steps(lat, steps, warmup):
     return wall, kern
 
 
-def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015, store_range=1.0):
+def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015, store_range=1.0, ceiling=True):
     import torch
     from pytnl_lbm import D3Q27Lattice, _abi
     from pytnl_lbm.fields import turbulence_fields_torch
@@ -115,6 +116,19 @@ def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015
     red = lat.reduce_macro()
     res["mass_rel_drift"] = abs(red[0] / sites - 1.0)
     res["nan_sites"] = red[2]
+    if ceiling:
+        # the same kernels, launch shape and allocation with the identity
+        # collision: the memory ceiling of the step's access pattern here
+        _, ck = timed_steps(lat, steps, 2, fn=lat.stream_only)
+        c_avg = sum(ck) / len(ck)
+        res["ceiling"] = {
+            "what": "tnl_lbm_d3q27_stream_only: the step kernels with the identity collision (free streaming), "
+                    "same lattice allocation, same bytes",
+            "kernel_ms_avg": round(c_avg * 1e3, 3),
+            "kernel_ms_even": round(sum(ck[0::2]) / max(1, len(ck[0::2])) * 1e3, 3),
+            "kernel_ms_odd": round(sum(ck[1::2]) / max(1, len(ck[1::2])) * 1e3, 3),
+            "roofline_frac": round(bytes_per_site * sites / c_avg / 1e9 / PEAK_HBM_GBS, 4),
+            "step_over_ceiling": round(c_avg / avg_kernel, 4)}
     del lat
     torch.cuda.empty_cache()
     return res
@@ -514,6 +528,7 @@ def main():
                      "kernel_ms_even": round(head["kernel_ms_even"], 3),
                      "kernel_ms_odd": round(head["kernel_ms_odd"], 3)},
         "checks": {"mass_rel_drift": head["mass_rel_drift"], "nan_sites": head["nan_sites"]},
+        "ceiling": head.get("ceiling"),
     }
     progress(f"{workload}: {head['mlups']:.0f} MLUPS, {100 * head['roofline_frac']:.1f}% of 8 TB/s")
     if not args.no_shard and st == "f64" and N % args.shard_slab == 0:
@@ -529,7 +544,8 @@ def main():
                                  "roofline_frac": round(r["roofline_frac"], 4), "bytes_per_site": r["bytes_per_site"],
                                  "kernel_ms_avg": round(r["kernel_ms_avg"], 3),
                                  "kernel_ms_even": round(r["kernel_ms_even"], 3),
-                                 "kernel_ms_odd": round(r["kernel_ms_odd"], 3)}
+                                 "kernel_ms_odd": round(r["kernel_ms_odd"], 3),
+                                 "ceiling": r.get("ceiling")}
             progress(f"{pst} {P}^3: {r['mlups']:.0f} MLUPS, {100 * r['roofline_frac']:.1f}% of 8 TB/s")
         spec = []
         for ic in ("tgv", "turbulence"):

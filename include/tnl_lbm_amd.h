@@ -235,12 +235,31 @@ typedef struct tnl_lbm_d3q7_data {
 uint32_t tnl_lbm_abi_version(void);
 const char* tnl_lbm_last_error(void);
 
+/* No counterpart in the reference: a diagnostic switch.  Where a population
+ * array holds at most 4 GiB, the map kernel (2-byte forms), the coupled
+ * kernel (2-byte forms), the recording step and the FP64 adjoint address
+ * each slot as a per-direction base plus a 32-bit offset; on = 1 makes every
+ * launch of this process take the 64-bit form (same results, bit for bit),
+ * on = 0 restores the default.  Returns the previous setting.              */
+int tnl_lbm_set_wide_addressing(int on);
+
 /* ---- D3Q27 lattice update ----------------------------------------------
  * Replaces one launch of cudaLBMKernel<NSE>(SD, offset, end, distributed,
  * output_macro) (kernels.h:59-112, launched at state.hpp:1307-1309) over the
  * box offset <= (x,y,z) < end.  flags: TNL_LBM_FLAG_*.                     */
 int tnl_lbm_d3q27_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD,
                        const int64_t offset[3], const int64_t end[3], uint32_t flags, void* stream);
+
+/* ---- streaming-only measurement step -----------------------------------
+ * No counterpart in the reference: a measurement aid.  One A-A step of free
+ * streaming (the collision replaced by the identity) over the whole box,
+ * run by the same map-free periodic kernels, loads, stores and launch shape
+ * as tnl_lbm_d3q27_step for this storage form and SD->even_iter: its time is
+ * the memory ceiling of the step's access pattern on this allocation.  Even
+ * step: slot opp(q) of each site takes slot q; odd step: slot q at r + c_q
+ * takes slot opp(q) at r - c_q (two steps move population q by 2 c_q).
+ * Needs pattern A-A, no map and no force field.                            */
+int tnl_lbm_d3q27_stream_only(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, void* stream);
 
 /* ---- coupled NSE + ADE update ------------------------------------------
  * Replaces cudaLBMKernel<NSE,ADE> (kernels.h:114-196): the D3Q27 step, then

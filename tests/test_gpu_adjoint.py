@@ -202,34 +202,46 @@ def test_ab_adjoint_differentiates_the_aa_run():
     the gradient of the A-A run: dJ/d(profile) of the A-B run against central
     differences of an A-A run of the same duct, J from the A-A step's macro
     output at step T (the pre-collision rho, u of that step, which the
-    collision conserves).  INFLOW in, OUTFLOW_EQ out: the zero-gradient
-    OUTFLOW_RIGHT reads the column behind it after streaming and exists for
-    the pull scheme only."""
+    collision conserves).  INFLOW in, OUTFLOW_EQ out on x = X-2 with a wall
+    column behind it: the zero-gradient OUTFLOW_RIGHT reads the column behind
+    it after streaming and exists for the pull scheme only, and an outlet on
+    the array's last plane would pull from outside the box, which A-A wraps
+    (to the inlet) and A-B clamps (to itself), so the two runs would differ
+    there.  The wall sends back only what it pulled from the fluid side."""
     from pytnl_lbm import CheckpointedRun, channel_geometry, density_velocity, inflow_control, lattice_run, populations
     X, Y, Z, T, xp = 32, 16, 16, 60, 24
     geo = channel_geometry(X, Y, Z, outflow=GEO.OUTFLOW_EQ)
+    geo[-1] = GEO.WALL
+    geo[-2, 1:-1, 1:-1] = GEO.OUTFLOW_EQ
     nu = (0.51 - 0.5) / 3
     ab = D3Q27Lattice(X, Y, Z, pattern="AB", nu=nu, geo=geo)
     ab.set_equilibrium(1.0, 0.02, 0.0, 0.0)
     f0 = ab.df[0].clone()
     run = CheckpointedRun(ab, T)
     p = torch.tensor(inflow_control(Y, Z, 0.04, 0.005, seed=1), device="cuda")
-    ut = torch.full((3, Z, Y), 0.01, dtype=torch.float64, device="cuda")
+    ut = torch.full((3, Z - 2, Y - 2), 0.01, dtype=torch.float64, device="cuda")
     pr = p.clone().requires_grad_()
-    _, u = density_velocity(populations(lattice_run(run, pr, f0), ab)[:, xp])
+    # fluid sites of the plane only: the wall sites' stored populations are
+    # bounced back, so their moments are not the macros of the step
+    _, u = density_velocity(populations(lattice_run(run, pr, f0), ab)[:, xp, 1:-1, 1:-1])
     (0.5 * ((u - ut) ** 2).sum()).backward()
 
-    def J_aa(pp):
-        aa = D3Q27Lattice(X, Y, Z, pattern="AA", nu=nu, geo=geo)
-        aa.set_equilibrium(1.0, 0.02, 0.0, 0.0)
-        aa.set_inflow(0.0, 0.0, 0.0, profile=[pp[0], pp[1], pp[2]])
+    def u_plane(pattern, pp):
+        lat = D3Q27Lattice(X, Y, Z, pattern=pattern, nu=nu, geo=geo)
+        lat.set_equilibrium(1.0, 0.02, 0.0, 0.0)
+        lat.set_inflow(0.0, 0.0, 0.0, profile=[pp[0], pp[1], pp[2]])
         for _ in range(T - 1):
-            aa.step()
-        aa.step(output_macro=True)  # T is even: the A-A populations are in place again
-        rho, ux, uy, uz = aa.macros()
-        u_aa = torch.stack([ux[xp], uy[xp], uz[xp]]).permute(0, 2, 1)
-        return 0.5 * ((u_aa - ut) ** 2).sum().item()
+            lat.step()
+        lat.step(output_macro=True)  # T is even: the A-A populations are in place again
+        rho, ux, uy, uz = lat.macros()
+        return torch.stack([ux[xp], uy[xp], uz[xp]]).permute(0, 2, 1)[:, 1:-1, 1:-1]
 
+    def J_aa(pp):
+        return 0.5 * ((u_plane("AA", pp) - ut) ** 2).sum().item()
+
+    # the premise: the A-A run's macros at step T are the A-B run's
+    assert (u_plane("AA", p) - u_plane("AB", p)).abs().max().item() <= 1e-14
+    assert (u_plane("AB", p) - u).abs().max().item() <= 1e-13
     eps = 1e-4
     gen = torch.Generator(device="cuda").manual_seed(8)
     for _ in range(3):

@@ -529,3 +529,41 @@ def test_site_force_field_without_map_runs_the_map_kernel(storage, tol):
     for t in range(5):
         g2.step()
     assert rel_max(g2.export_f64().cpu().numpy() - Wb, h.export_f64() - Wb) > 10 * tol
+
+
+def _raw_populations(storage, n, seed):
+    """Random stored values of a storage form whose decode/encode round trip is exact."""
+    g = torch.Generator().manual_seed(seed)
+    v = torch.rand(n, generator=g, dtype=torch.float64) * 0.2 - 0.1
+    if storage == "i16dev":
+        return torch.randint(-32767, 32768, (n,), generator=g, dtype=torch.int16)
+    return v.to({"f64": torch.float64, "f32": torch.float32, "f32dev": torch.float32, "f16dev": torch.float16,
+                 "bf16dev": torch.bfloat16}[storage])
+
+
+@pytest.mark.parametrize("storage", list(STORE))
+@pytest.mark.parametrize("shape", [(3, 512, 4), (4, 70, 3)])
+def test_stream_only_is_free_streaming(storage, shape):
+    """tnl_lbm_d3q27_stream_only runs the periodic step kernels with the
+    identity collision: the even step moves slot q of each site to slot
+    opp(q), the odd step then leaves population q displaced by 2 c_q, bit for
+    bit (512-site rows take the 512-lane and paired shapes, 70-site rows the
+    256-lane one)."""
+    X, Y, Z = shape
+    g = D3Q27Lattice(X, Y, Z, storage=storage, nu=0.01, store_range=0.5)
+    f0 = _raw_populations(storage, g.df[0].numel(), 5)
+    g.df[0].copy_(f0.to(g.df[0].device))
+    s0 = g.stored().clone()
+    g.stream_only()
+    torch.cuda.synchronize()
+    s1 = g.stored()
+    bits = lambda t: t.view(torch.int16) if t.element_size() == 2 else t  # noqa: E731
+    for q in range(27):
+        assert torch.equal(bits(s1[OPP[q]]), bits(s0[q])), (storage, shape, "even", q)
+    g.stream_only()
+    torch.cuda.synchronize()
+    s2 = g.stored().cpu()
+    s0 = s0.cpu()
+    for q in range(27):
+        want = torch.roll(s0[q], shifts=(2 * int(CX[q]), 2 * int(CZ[q]), 2 * int(CY[q])), dims=(0, 1, 2))
+        assert torch.equal(bits(s2[q]), bits(want)), (storage, shape, "odd", q)

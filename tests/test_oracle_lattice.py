@@ -36,6 +36,38 @@ def test_aa_of_streamed_state_equals_ab(coll):
             assert rel_max(a, b) < 1e-14
 
 
+def _duct_macros(pattern, geo, T, profile):
+    X, Y, Z = geo.shape
+    h = O.HostLattice(X, Y, Z, pattern=pattern, nu=(0.51 - 0.5) / 3, geo=geo)
+    h.set_equilibrium(1.0, 0.02, 0.0, 0.0)
+    h.set_inflow(0.0, 0.0, 0.0, profile=list(profile))
+    for t in range(T):
+        h.step(output_macro=(t == T - 1))
+    return [np.array(m) for m in h.macros()]
+
+
+def test_aa_equals_ab_on_a_duct_whose_out_of_box_pulls_stay_outside():
+    """On a duct with walls, an equilibrium inlet and an OUTFLOW_EQ outlet,
+    A-A and A-B give the same macros from a uniform state (its own stream)
+    when a wall column stands behind the outlet: the sites on the array's
+    faces then send back into the box only what they pulled from inside it.
+    With the outlet on the last plane itself the runs differ, because its
+    pulls from beyond the box wrap in A-A (to the inlet plane) and clamp in
+    A-B (kernels.h:5-57)."""
+    from pytnl_lbm import channel_geometry, inflow_control
+    X, Y, Z, T = 14, 8, 8, 16
+    prof = inflow_control(Y, Z, 0.04, 0.005, seed=1)
+    edge = channel_geometry(X, Y, Z, outflow=GEO.OUTFLOW_EQ)
+    behind = edge.copy()
+    behind[-1] = GEO.WALL
+    behind[-2, 1:-1, 1:-1] = GEO.OUTFLOW_EQ
+    for a, b in zip(_duct_macros(_abi.PATTERN_AB, behind, T, prof), _duct_macros(_abi.PATTERN_AA, behind, T, prof)):
+        assert np.array_equal(a, b)
+    d = max(np.abs(a - b).max() for a, b in zip(_duct_macros(_abi.PATTERN_AB, edge, T, prof),
+                                                 _duct_macros(_abi.PATTERN_AA, edge, T, prof)))
+    assert d > 1e-6
+
+
 def test_aa_population_bookkeeping():
     """After an even step the A-A array holds post-collision values in the
     opposite slots of the same site; after the odd step they are streamed."""

@@ -17,7 +17,7 @@ int adjoint_for(const StepArgs& A0, const AdjPtrs& P, unsigned threads, unsigned
 	// -68 VALU, ~2% faster on the C4 duct; FP32 measured ~2% slower with them
 	bool o32 = false;
 	if constexpr (ST == TNL_LBM_STORE_F64)
-		o32 = (uint64_t) A.XYZ * sizeof(double) <= (1ull << 32);
+		o32 = narrow_slots(A.XYZ, sizeof(double));
 	if (o32) {
 		if constexpr (ST == TNL_LBM_STORE_F64)
 			hipLaunchKernelGGL((k_adjoint_ab<ST, false, TAPE, true>), dim3(grid), dim3(threads), 0, st, A, P);
@@ -39,7 +39,7 @@ int record_for(const StepArgs& A0, unsigned threads, unsigned grid, hipStream_t 
 	linearize(A, threads, grid);
 	// 32-bit slot offsets where each population array holds at most 4 GiB:
 	// -124 (FP64) / -151 (FP32) VALU, ~4% on the C4 duct (kernel trace)
-	if ((uint64_t) A.XYZ * sizeof(typename Storage<ST>::elem) <= (1ull << 32))
+	if (narrow_slots(A.XYZ, sizeof(typename Storage<ST>::elem)))
 		hipLaunchKernelGGL((k_general<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, TAPE, true>), dim3(grid), dim3(threads), 0, st, A);
 	else
 		hipLaunchKernelGGL((k_general<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, TAPE>), dim3(grid), dim3(threads), 0, st, A);

@@ -15,6 +15,7 @@
 //    path reads the int16 map and applies D3Q27_BC_All (d3q27/bc.h:61-422).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -40,6 +41,12 @@ int hip_check(hipError_t e, const char* what)
 	if (e != hipSuccess)
 		return fail(TNL_LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 	return TNL_LBM_OK;
+}
+
+static std::atomic<int> g_wide_addressing{0};
+bool narrow_slots_allowed()
+{
+	return g_wide_addressing.load(std::memory_order_relaxed) == 0;
 }
 
 }  // namespace lbm
@@ -210,6 +217,11 @@ const char* tnl_lbm_last_error(void)
 	return g_err.c_str();
 }
 
+int tnl_lbm_set_wide_addressing(int on)
+{
+	return g_wide_addressing.exchange(on ? 1 : 0);
+}
+
 static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t offset[3],
 						  const int64_t end[3], uint32_t flags, StepArgs& A, unsigned& threads, unsigned& grid);
 
@@ -224,6 +236,27 @@ int tnl_lbm_d3q27_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, 
 		return TNL_LBM_OK;
 	hipStream_t st = (hipStream_t) stream;
 	STORAGE_SWITCH(cfg->storage, (dispatch_coll<ST>(cfg, A, threads, grid, st)));
+}
+
+int tnl_lbm_d3q27_stream_only(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, void* stream)
+{
+	if (! cfg || ! SD)
+		return fail(TNL_LBM_ERR_NULL, "null config or data pointer");
+	if (cfg->pattern != TNL_LBM_PATTERN_AA || SD->map || SD->force_field)
+		return fail(TNL_LBM_ERR_UNSUPPORTED, "streaming-only runs the map-free periodic A-A kernels only");
+	const int64_t offset[3] = {0, 0, 0};
+	const int64_t end[3] = {SD->X, SD->Y, SD->Z};
+	StepArgs A;
+	unsigned threads = 0, grid = 0;
+	if (int rc = make_step_args(cfg, SD, offset, end, 0, A, threads, grid))
+		return rc;
+	if (grid == 0)
+		return TNL_LBM_OK;
+	// the fixed-point encode without the fast collision's 2^-n output factor
+	for (int n = 0; n < 4; n++)
+		A.cls_inv[n] = A.cls_sc[n] > 0 ? (float) (1.0 / ((double) A.cls_sc[n] * 32767.0)) : 0.0f;
+	hipStream_t st = (hipStream_t) stream;
+	STORAGE_SWITCH(cfg->storage, (launch_periodic<ST, COLL_STREAM_ONLY>(A, threads, grid, st)));
 }
 
 static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t offset[3],
@@ -353,7 +386,7 @@ int tnl_lbm_nse_ade_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* ns
 		// 2-byte forms: 32-bit slot offsets where each population array (D3Q27
 		// and D3Q7 share XYZ) holds at most 4 GiB, as in the map kernel
 		constexpr bool two = sizeof(typename Storage<ST>::elem) == 2;
-		if (two && (uint64_t) A.XYZ * 2 <= (1ull << 32)) {
+		if (two && narrow_slots(A.XYZ, 2)) {
 			if constexpr (two) {
 				if (cfg->pattern == TNL_LBM_PATTERN_AA)
 					hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA, true>), dim3(grid), dim3(threads), 0, st, A, B);

@@ -18,6 +18,17 @@ namespace lbm {
 // host-side error reporting (defined in lbm_d3q27.hip)
 int hip_check(hipError_t e, const char* what);
 
+// Whether the kernels that can address a population array of at most 4 GiB
+// as a per-direction base plus a 32-bit offset may do so; false while
+// tnl_lbm_set_wide_addressing(1) holds (defined in lbm_d3q27.hip).
+bool narrow_slots_allowed();
+
+// a population array of XYZ elements of `elem` bytes takes 32-bit slot offsets
+inline bool narrow_slots(int64_t XYZ, size_t elem)
+{
+	return narrow_slots_allowed() && (uint64_t) XYZ * elem <= (1ull << 32);
+}
+
 // ---------------------------------------------------------------------------
 struct StepArgs
 {
@@ -182,6 +193,12 @@ LBM_DEV void collide(typename Storage<ST>::real* f, typename Storage<ST>::real r
 template <int ST, int COLL>
 constexpr bool fast_collision = std::is_same<typename Storage<ST>::real, float>::value && COLL == TNL_LBM_COLL_CUM;
 
+// Collision slot of the streaming-only measurement kernel
+// (tnl_lbm_d3q27_stream_only): the identity, so that the periodic kernels
+// move the populations of free streaming with the step's own loads, stores
+// and addressing, and time the memory traffic of the access pattern alone.
+constexpr int COLL_STREAM_ONLY = 1000;
+
 // number of non-zero components of c_q (the weight class of q)
 constexpr int ncls(int q)
 {
@@ -201,7 +218,10 @@ LBM_DEV void collide_site(typename Storage<ST>::real* f, const StepArgs& A, type
 						  typename Storage<ST>::real& vx, typename Storage<ST>::real& vy, typename Storage<ST>::real& vz)
 {
 	using real = typename Storage<ST>::real;
-	if constexpr (fast_collision<ST, COLL>) {
+	if constexpr (COLL == COLL_STREAM_ONLY) {
+		rho = vx = vy = vz = 0;
+	}
+	else if constexpr (fast_collision<ST, COLL>) {
 		FastCum<Storage<ST>::dev>::collide(f, (real) A.fx, (real) A.fy, (real) A.fz, (real) A.omega1, true, rho, vx, vy, vz);
 	}
 	else {
@@ -1430,35 +1450,43 @@ inline bool pair_shape(const StepArgs& A, StepArgs& B, unsigned& threads, unsign
 	return true;
 }
 
+// The map-free periodic A-A kernels: the paired even step for the 2-byte
+// forms where the row allows it, else one site per lane.
+template <int ST, int COLL>
+int launch_periodic(const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
+{
+	if constexpr (sizeof(typename Storage<ST>::elem) == 2 && ((LBM_PAIR_EVEN >> ST) & 1)) {
+		StepArgs B;
+		unsigned t2 = 0, g2 = 0;
+		if (A.even && pair_shape(A, B, t2, g2)) {
+			hipLaunchKernelGGL((k_aa_even_pair<ST, COLL>), dim3(g2), dim3(t2), 0, st, B);
+			return hip_check(hipGetLastError(), "step launch");
+		}
+	}
+	if (ST != TNL_LBM_STORE_F64 && LBM_TPB_FP32 > 256 && A.ny % LBM_TPB_FP32 == 0) {
+		StepArgs B = A;
+		B.nyb = (uint32_t) A.ny / LBM_TPB_FP32;
+		B.nblocks = B.nyb * (uint32_t) A.nz * (uint32_t) A.nx;  // fewer blocks than at 256
+		const uint32_t padded = (B.nblocks + NXCD - 1) / NXCD * NXCD;
+		B.per_xcd = padded / NXCD;
+		if (A.even)
+			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(padded), dim3(LBM_TPB_FP32), 0, st, B);
+		else
+			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(padded), dim3(LBM_TPB_FP32), 0, st, B);
+	}
+	else if (A.even)
+		hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(grid), dim3(threads), 0, st, A);
+	else
+		hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(grid), dim3(threads), 0, st, A);
+	return hip_check(hipGetLastError(), "step launch");
+}
+
 template <int ST, int COLL>
 int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
 {
 	// the map-free periodic kernels take the homogeneous force only
-	if (! A.map && ! A.force && cfg->pattern == TNL_LBM_PATTERN_AA) {
-		if constexpr (sizeof(typename Storage<ST>::elem) == 2 && ((LBM_PAIR_EVEN >> ST) & 1)) {
-			StepArgs B;
-			unsigned t2 = 0, g2 = 0;
-			if (A.even && pair_shape(A, B, t2, g2)) {
-				hipLaunchKernelGGL((k_aa_even_pair<ST, COLL>), dim3(g2), dim3(t2), 0, st, B);
-				return hip_check(hipGetLastError(), "step launch");
-			}
-		}
-		if (ST != TNL_LBM_STORE_F64 && LBM_TPB_FP32 > 256 && A.ny % LBM_TPB_FP32 == 0) {
-			StepArgs B = A;
-			B.nyb = (uint32_t) A.ny / LBM_TPB_FP32;
-			B.nblocks = B.nyb * (uint32_t) A.nz * (uint32_t) A.nx;  // fewer blocks than at 256
-			const uint32_t padded = (B.nblocks + NXCD - 1) / NXCD * NXCD;
-			B.per_xcd = padded / NXCD;
-			if (A.even)
-				hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(padded), dim3(LBM_TPB_FP32), 0, st, B);
-			else
-				hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(padded), dim3(LBM_TPB_FP32), 0, st, B);
-		}
-		else if (A.even)
-			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, true>), dim3(grid), dim3(threads), 0, st, A);
-		else
-			hipLaunchKernelGGL((k_aa_periodic<ST, COLL, false>), dim3(grid), dim3(threads), 0, st, A);
-	}
+	if (! A.map && ! A.force && cfg->pattern == TNL_LBM_PATTERN_AA)
+		return launch_periodic<ST, COLL>(A, threads, grid, st);
 	else {
 		StepArgs B = A;
 		linearize(B, threads, grid);
@@ -1467,7 +1495,7 @@ int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, 
 		// A-B -2%; neutral for the 4- and 8-byte forms, which keep 64-bit)
 		bool o32 = false;
 		if constexpr (sizeof(typename Storage<ST>::elem) == 2)
-			o32 = (uint64_t) A.XYZ * sizeof(typename Storage<ST>::elem) <= (1ull << 32);
+			o32 = narrow_slots(A.XYZ, sizeof(typename Storage<ST>::elem));
 		if (o32) {
 			if constexpr (sizeof(typename Storage<ST>::elem) == 2) {
 				if (cfg->pattern == TNL_LBM_PATTERN_AA)

@@ -176,6 +176,7 @@ def _declare(lib, prefix: str):
     cfgp, d27p, d7p = C.POINTER(Config), C.POINTER(D3Q27Data), C.POINTER(D3Q7Data)
     sig = {
         "d3q27_step": [cfgp, d27p, I64x3, I64x3, C.c_uint32],
+        "d3q27_stream_only": [cfgp, d27p],
         "nse_ade_step": [cfgp, d27p, d7p, I64x3, I64x3, C.c_uint32],
         "d3q27_set_equilibrium": [cfgp, d27p, C.c_double, C.c_double, C.c_double, C.c_double],
         "d3q27_set_equilibrium_field": [cfgp, d27p, _P, _P, _P, _P],
@@ -227,6 +228,8 @@ def load_library():
     _declare(lib, "tnl_lbm_")
     lib.tnl_lbm_abi_version.restype = C.c_uint32
     lib.tnl_lbm_last_error.restype = C.c_char_p
+    lib.tnl_lbm_set_wide_addressing.argtypes = [C.c_int]
+    lib.tnl_lbm_set_wide_addressing.restype = C.c_int
     lib.tnl_lbm_d3q27_reduce_macro.argtypes = [C.POINTER(Config), C.POINTER(D3Q27Data),
                                                C.POINTER(C.c_double), _P]
     lib.tnl_lbm_d3q27_reduce_macro.restype = C.c_int

@@ -196,6 +196,14 @@ class D3Q27Lattice:
                                                 _abi.I64x3(*end), fl, _stream_ptr(stream)))
         self.iterations += 1
 
+    def stream_only(self, stream=None):
+        """One A-A step of free streaming (the collision replaced by the
+        identity) through the periodic step kernels: a measurement of the
+        step's memory ceiling on this allocation (tnl_lbm_d3q27_stream_only)."""
+        self.update_kernel_data()
+        self._check(self.lib.tnl_lbm_d3q27_stream_only(C.byref(self.cfg), C.byref(self.d), _stream_ptr(stream)))
+        self.iterations += 1
+
     def step_coupled(self, ade: "D3Q7Lattice", output_macro=False, stream=None, flags=0, offset=(0, 0, 0), end=None):
         """One cudaLBMKernel<NSE,ADE> launch (kernels.h:114-196, state_NSE_ADE.h:201-210): the D3Q27
         update and, at the same site, the D3Q7 update driven by the NSE velocity."""
