@@ -100,6 +100,21 @@ def test_checkpoint_round_trip_with_reference_names(tmp_path):
         "LBM_iterations"] == a.iterations - 5
 
 
+def test_checkpoint_restores_the_map_into_a_lattice_without_one(tmp_path):
+    """A State resuming a run has not built its map when it loads the
+    checkpoint: the saved map is created on the lattice; a run saved without
+    a map comes back without one."""
+    a, b = FakeBlock(seed=1), FakeBlock(seed=2)
+    b.map = None
+    lio.save_checkpoint(tmp_path / "c", [a])
+    lio.load_checkpoint(tmp_path / "c", [b])
+    assert b.map is not None and torch.equal(a.map, b.map)
+    a.map = None
+    lio.save_checkpoint(tmp_path / "d", [a])
+    lio.load_checkpoint(tmp_path / "d", [b])
+    assert b.map is None
+
+
 def test_checkpoint_refuses_a_different_lattice(tmp_path):
     a = FakeBlock()
     lio.save_checkpoint(tmp_path / "c", [a])

@@ -300,8 +300,17 @@ def load_checkpoint(path, blocks, *, ade_blocks=(), state=None) -> dict:
             for key in ("X", "Y", "Z", "overlaps", "XYZ", "storage", "pattern", "macro_real", "collision"):
                 if saved[key] != mine[key]:
                     raise ValueError(f"{path}: {prefix} block {bid} has {key} = {saved[key]}, the lattice {mine[key]}")
-            if lat.map is not None:
-                load(f"{prefix}_map_block_{bid}", lat.map)
+            # the saved geometry, also into a lattice that has no map yet (a
+            # State resuming from a checkpoint allocates none before loading)
+            mname = f"{prefix}_map_block_{bid}"
+            if mname in attrs["variables"]:
+                if lat.map is None:
+                    import torch
+                    lat.map = torch.empty(attrs["variables"][mname]["size"], dtype=torch.int16,
+                                          device=lat.df[0].device)
+                load(mname, lat.map)
+            else:
+                lat.map = None
             for k, t in enumerate(lat.df):
                 load(f"{prefix}_df_{k}_block_{bid}", t)
             if prefix == "LBM":
