@@ -54,6 +54,10 @@ def parse():
     p.add_argument("--storage", default="f64", choices=["f64", "f32", "f32dev", "f16dev", "bf16dev", "i16dev"],
                    help="storage form of the headline case (f64 is the headline workload; the others exist "
                         "so that each kernel can be profiled through this script)")
+    p.add_argument("--allocations", type=int, default=1,
+                   help="also time the headline step and its streaming-only ceiling on this many further "
+                        "allocations of the headline box, each made while the previous one is still held "
+                        "(the achieved fraction depends on where the driver places the allocation)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain device copy of 2 GiB (the known-size kernel that scales the "
                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -173,6 +177,34 @@ def shard_case(N, slab, steps, warmup, tau=0.5015):
     del lat
     torch.cuda.empty_cache()
     return res
+
+
+def placement_survey(N, storage, count, steps=6, tau=0.5015):
+    """The headline step and the streaming-only step on `count` successive
+    allocations of the N^3 box, each made while the previous one is held (so
+    the driver places it elsewhere); uniform initial state.  Per allocation:
+    fraction of 8 TB/s for both, and their ratio."""
+    import torch
+    from pytnl_lbm import D3Q27Lattice, _abi
+
+    out, held = [], None
+    for i in range(count):
+        lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0)
+        lat.set_equilibrium(1.0, 0.02, -0.01, 0.015)
+        _, k = timed_steps(lat, steps, 2)
+        _, c = timed_steps(lat, steps, 2, fn=lat.stream_only)
+        gb = 2 * 27 * _abi.STORAGE_BYTES[lat.storage] * N ** 3 / 1e9
+        ks, cs = sum(k) / len(k), sum(c) / len(c)
+        out.append({"allocation": i + 1, "step_frac": round(gb / ks / PEAK_HBM_GBS, 4),
+                    "ceiling_frac": round(gb / cs / PEAK_HBM_GBS, 4), "step_over_ceiling": round(cs / ks, 4)})
+        progress(f"allocation {i + 1}: step {out[-1]['step_frac']}, ceiling {out[-1]['ceiling_frac']}")
+        del held
+        torch.cuda.empty_cache()
+        held = lat
+        del lat
+    del held
+    torch.cuda.empty_cache()
+    return out
 
 
 def precision_errors(N, steps, storages, ic="tgv"):
@@ -531,6 +563,8 @@ def main():
         "ceiling": head.get("ceiling"),
     }
     progress(f"{workload}: {head['mlups']:.0f} MLUPS, {100 * head['roofline_frac']:.1f}% of 8 TB/s")
+    if args.allocations > 1:
+        out["placement_survey"] = placement_survey(N, st, args.allocations - 1)
     if not args.no_shard and st == "f64" and N % args.shard_slab == 0:
         out["shard"] = shard_case(N, args.shard_slab, min(args.steps, 40), 2)
         progress(f"shard {out['shard']['workload']}: {out['shard']['mlups']:.0f} MLUPS, "

@@ -567,3 +567,37 @@ def test_stream_only_is_free_streaming(storage, shape):
     for q in range(27):
         want = torch.roll(s0[q], shifts=(2 * int(CX[q]), 2 * int(CZ[q]), 2 * int(CY[q])), dims=(0, 1, 2))
         assert torch.equal(bits(s2[q]), bits(want)), (storage, shape, "odd", q)
+
+
+@pytest.mark.parametrize("storage", ["f64", "i16dev"])
+def test_turbulence_box_at_the_headline_relaxation_matches_oracle(storage):
+    """The headline workload's initial condition and relaxation (seeded
+    solenoidal turbulence, k0 = 8, u_rms = 0.04, tau = 0.5015, A-A periodic)
+    at 32^3: FP64 per step to 1e-13 of the oracle; the I16 form (per-class
+    bounds from the field's velocity bound) within a few of its coarsest
+    quanta of the oracle's fixed-point chain, as the deviation-storage test
+    holds it."""
+    from pytnl_lbm import i16_class_ranges
+    from pytnl_lbm.fields import turbulence_fields
+    N = 32
+    r = turbulence_fields(N, 1, 8.0, 0.04)  # rho = 1, vx, vy, vz
+    umax = float(np.sqrt(sum(c ** 2 for c in r[1:])).max())
+    Rc = i16_class_ranges(umax) if storage == "i16dev" else None
+    g, h = pair(N, N, N, storage=storage, nu=(0.5015 - 0.5) / 3.0, store_range_class=Rc)
+    g.set_equilibrium_field(*r)
+    h.set_equilibrium_field(*r)
+    Wb = W[:, None, None, None]
+    if Rc is not None:
+        ncls = (CX != 0).astype(int) + (CY != 0) + (CZ != 0)
+        quantum = (W * np.array(Rc)[ncls] / 32767)[:, None, None, None]
+    for t in range(6):
+        g.step(output_macro=True)
+        h.step(output_macro=True)
+        torch.cuda.synchronize()
+        if storage == "f64":
+            assert cmp_pops(g, h) < 1e-13, t
+        else:
+            diff = np.abs((g.export_f64().cpu().numpy() - Wb) - (h.export_f64() - Wb))
+            assert diff.max() <= 4 * quantum.max(), (t, diff.max())
+            assert np.mean(diff > 0.5 * quantum) < 0.02, t
+    assert np.isfinite(g.macros()[1].cpu().numpy()).all()
