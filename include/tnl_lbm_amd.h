@@ -373,7 +373,9 @@ int tnl_lbm_d3q27_halo_x(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* lo
  *   lambda      <- (d df_out / d df[0])^T lambda_next
  *   grad_inflow += (d df_out / d inflow velocity)^T lambda_next
  * lambda_next / lambda: 27*XYZ elements of the storage type (F64 or F32),
- * same layout as the populations; lambda is overwritten, must not alias
+ * same layout as the populations (F32: float fields, double arithmetic, so
+ * the rounding of long adjoint runs stays that of the stored fields); lambda
+ * is overwritten, must not alias
  * lambda_next or df[0].  grad_inflow: 3*Y*Z doubles (component-major, index
  * Y*z + y as the inflow profiles) accumulated, or NULL.  Geometry codes with
  * an adjoint: FLUID, PERIODIC, WALL, INFLOW, OUTFLOW_EQ, OUTFLOW_RIGHT,

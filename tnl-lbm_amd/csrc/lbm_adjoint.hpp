@@ -49,7 +49,8 @@ struct AdjCum
 	// populations f.  rho_fixed: the caller has replaced rho by 1 before the
 	// collision (OUTFLOW_RIGHT, bc.h:191-194): the 4th/6th-order products then
 	// divide by 1 while u and the trace keep the actual mass.
-	static LBM_DEV void vjp(const real* f, const real* lam, real Fx, real Fy, real Fz, real omega1, bool rho_fixed, real* mu)
+	template <typename L, typename Mu>
+	static LBM_DEV void vjp(const real* f, const L* lam, real Fx, real Fy, real Fz, real omega1, bool rho_fixed, Mu* mu)
 	{
 		real m[NMOM];
 		raw_moments(f, m);
@@ -57,8 +58,12 @@ struct AdjCum
 	}
 
 	// the same from the raw moments of the pre-collision populations, which
-	// are all the Jacobian of the default cumulant depends on
-	static LBM_DEV void vjp_m(const real* m, const real* lam, real Fx, real Fy, real Fz, real omega1, bool rho_fixed, real* mu)
+	// are all the Jacobian of the default cumulant depends on.  lam and mu may
+	// be held in a narrower type than the arithmetic (float fields, double
+	// sweep): each value is widened where it is read and rounded once where
+	// it is written.
+	template <typename L, typename Mu>
+	static LBM_DEV void vjp_m(const real* m, const L* lam, real Fx, real Fy, real Fz, real omega1, bool rho_fixed, Mu* mu)
 	{
 		// ---- forward recomputation ---------------------------------------
 		const real m0 = m[0], jx = m[1], jy = m[2], jz = m[3], Pxx = m[4], Pyy = m[5], Pzz = m[6], Pxy = m[7], Pxz = m[8],
@@ -186,8 +191,8 @@ struct AdjCum
 #pragma unroll
 		for (int q = 0; q < 27; q++) {
 			const real cx = (real) CX[q], cy = (real) CY[q], cz = (real) CZ[q];
-			mu[q] = lm0 + cx * ljx + cy * ljy + cz * ljz + (cx * cx) * lk200 + (cy * cy) * lk020 + (cz * cz) * lk002
-				  + (cx * cy) * lk110 + (cx * cz) * lk101 + (cy * cz) * lk011;
+			mu[q] = (Mu) (lm0 + cx * ljx + cy * ljy + cz * ljz + (cx * cx) * lk200 + (cy * cy) * lk020 + (cz * cz) * lk002
+						  + (cx * cy) * lk110 + (cx * cz) * lk101 + (cy * cz) * lk011);
 		}
 	}
 
@@ -207,19 +212,19 @@ struct AdjCum
 	{
 		return (real) 6 * v + (real) (3 * c);
 	}
-	template <int q>
-	static LBM_DEV void eq_grad_q(const real* lam, real vx, real vy, real vz, real& gx, real& gy, real& gz)
+	template <int q, typename L>
+	static LBM_DEV void eq_grad_q(const L* lam, real vx, real vy, real vz, real& gx, real& gy, real& gz)
 	{
 		constexpr int nz = (CX[q] != 0) + (CY[q] != 0) + (CZ[q] != 0);
 		constexpr double coef = nz == 0 ? -1.0 / 27.0 : nz == 1 ? 1.0 / 54.0 : nz == 2 ? -1.0 / 108.0 : 1.0 / 216.0;
-		const real w = (real) coef * lam[q];
+		const real w = (real) coef * (real) lam[q];
 		const real px = phi<CX[q]>(vx), py = phi<CY[q]>(vy), pz = phi<CZ[q]>(vz);
 		gx += w * dphi<CX[q]>(vx) * py * pz;
 		gy += w * px * dphi<CY[q]>(vy) * pz;
 		gz += w * px * py * dphi<CZ[q]>(vz);
 	}
-	template <int q = 0>
-	static LBM_DEV void eq_grad(const real* lam, real vx, real vy, real vz, real& gx, real& gy, real& gz)
+	template <int q = 0, typename L>
+	static LBM_DEV void eq_grad(const L* lam, real vx, real vy, real vz, real& gx, real& gy, real& gz)
 	{
 		if constexpr (q < 27) {
 			eq_grad_q<q>(lam, vx, vy, vz, gx, gy, gz);
@@ -302,6 +307,9 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 	using real = typename S::real;
 	using elem = typename S::elem;
 	using D = AdjCum<real>;
+	// lambda in and mu out in the element type where it is narrower than the
+	// arithmetic (float fields with a double sweep): fewer registers
+	using lreal = std::conditional_t<(sizeof(elem) < sizeof(real)), elem, real>;
 	int64_t x, z, y;
 	if (SPECIAL ? ! shell_coords(A, x, z, y) : ! block_coords(A, x, z, y))
 		return;
@@ -345,19 +353,19 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 	// an irregular pull away from the faces (OUTFLOW_RIGHT inside the box) has no fix-up: poison it
 	const bool unsupported = gi != GEO_NOTHING && irregular != 0 && ! on_shell;
 
-	real mu[27];
+	lreal mu[27];
 	if (gi == GEO_NOTHING || unsupported) {
-		const real v = unsupported ? (real) __builtin_nan("") : (real) 0;
+		const lreal v = unsupported ? (lreal) __builtin_nan("") : (lreal) 0;
 #pragma unroll
 		for (int q = 0; q < 27; q++)
 			mu[q] = v;
 	}
 	else {
 		const elem* Ln = (const elem*) P.lam_next;
-		real lam[27];
+		lreal lam[27];
 #pragma unroll
 		for (int q = 0; q < 27; q++)
-			lam[q] = S::ld(slot0<O32, false>(Ln, q, A, s0));
+			lam[q] = (lreal) S::ld(slot0<O32, false>(Ln, q, A, s0));
 		// raw moments of the pulled populations: from the state, or recorded by the forward step
 		const bool needs_m = gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_EQ;
 		real m[D::NMOM];
@@ -422,14 +430,14 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 					D::eq_grad(lam, ux, uy, uz, gx, gy, gz);
 #pragma unroll
 					for (int q = 0; q < 27; q++)
-						mu[q] = (gx * ((real) CX[q] - ux) + gy * ((real) CY[q] - uy) + gz * ((real) CZ[q] - uz)) * rv;
+						mu[q] = (lreal) ((gx * ((real) CX[q] - ux) + gy * ((real) CY[q] - uy) + gz * ((real) CZ[q] - uz)) * rv);
 					break;
 				}
 			default:
 				// no adjoint for this geometry code: make the result unusable, visibly
 #pragma unroll
 				for (int q = 0; q < 27; q++)
-					mu[q] = (real) __builtin_nan("");
+					mu[q] = (lreal) __builtin_nan("");
 				break;
 		}
 	}

@@ -144,6 +144,20 @@ struct Storage<TNL_LBM_STORE_F32>
 	static LBM_DEV real ld(const elem* p, float = 1) { return *p; }
 	static LBM_DEV void st(elem* p, real v, float = 1) { *p = v; }
 };
+// Float elements with double arithmetic: a form of the FP32 adjoint step only
+// (its lambda fields and recordings stay float), not a storage form of the
+// lattice.
+constexpr int STORE_F32_F64MATH = 100;
+template <>
+struct Storage<STORE_F32_F64MATH>
+{
+	using elem = float;
+	using real = double;
+	static constexpr bool dev = false;
+	static constexpr bool fixed = false;
+	static LBM_DEV real ld(const elem* p, float = 1) { return (double) *p; }
+	static LBM_DEV void st(elem* p, real v, float = 1) { *p = (float) v; }
+};
 template <>
 struct Storage<TNL_LBM_STORE_F32_DEV>
 {
