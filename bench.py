@@ -132,7 +132,7 @@ def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015
             "kernel_ms_even": round(sum(ck[0::2]) / max(1, len(ck[0::2])) * 1e3, 3),
             "kernel_ms_odd": round(sum(ck[1::2]) / max(1, len(ck[1::2])) * 1e3, 3),
             "roofline_frac": round(bytes_per_site * sites / c_avg / 1e9 / PEAK_HBM_GBS, 4),
-            "step_over_ceiling": round(c_avg / avg_kernel, 4)}
+            "step_rate_over_ceiling_rate": round(c_avg / avg_kernel, 4)}
     del lat
     torch.cuda.empty_cache()
     return res
@@ -196,7 +196,7 @@ def placement_survey(N, storage, count, steps=6, tau=0.5015):
         gb = 2 * 27 * _abi.STORAGE_BYTES[lat.storage] * N ** 3 / 1e9
         ks, cs = sum(k) / len(k), sum(c) / len(c)
         out.append({"allocation": i + 1, "step_frac": round(gb / ks / PEAK_HBM_GBS, 4),
-                    "ceiling_frac": round(gb / cs / PEAK_HBM_GBS, 4), "step_over_ceiling": round(cs / ks, 4)})
+                    "ceiling_frac": round(gb / cs / PEAK_HBM_GBS, 4), "step_rate_over_ceiling_rate": round(cs / ks, 4)})
         progress(f"allocation {i + 1}: step {out[-1]['step_frac']}, ceiling {out[-1]['ceiling_frac']}")
         del held
         torch.cuda.empty_cache()

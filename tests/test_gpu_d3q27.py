@@ -573,10 +573,17 @@ def test_stream_only_is_free_streaming(storage, shape):
 def test_turbulence_box_at_the_headline_relaxation_matches_oracle(storage):
     """The headline workload's initial condition and relaxation (seeded
     solenoidal turbulence, k0 = 8, u_rms = 0.04, tau = 0.5015, A-A periodic)
-    at 32^3: FP64 per step to 1e-13 of the oracle; the I16 form (per-class
-    bounds from the field's velocity bound) within a few of its coarsest
-    quanta of the oracle's fixed-point chain, as the deviation-storage test
-    holds it."""
+    at 32^3: FP64 per step to 1e-13 of the oracle.  The I16 form (per-class
+    bounds from the field's velocity bound) is compared one step at a time
+    from the same state: after each step the oracle takes the GPU's
+    populations (exact multiples of the quanta, so its import re-quantises
+    them to the same integers), and the next step of both must agree to one
+    quantum of each slot's class, with few slots rounded apart.  Left to run
+    on its own, the oracle's chain drifts away by such flips: at tau =
+    0.5015 the collision scales the non-equilibrium parts by 1 - omega_1 =
+    -0.994, so a one-quantum flip is carried from step to step almost
+    undamped and spreads to the neighbours (8.7% of the slots after 6
+    steps)."""
     from pytnl_lbm import i16_class_ranges
     from pytnl_lbm.fields import turbulence_fields
     N = 32
@@ -597,12 +604,12 @@ def test_turbulence_box_at_the_headline_relaxation_matches_oracle(storage):
         if storage == "f64":
             assert cmp_pops(g, h) < 1e-13, t
         else:
-            diff = np.abs((g.export_f64().cpu().numpy() - Wb) - (h.export_f64() - Wb))
-            assert diff.max() <= 4 * quantum.max(), (t, diff.max())
-            # at tau = 0.5015 the collision scales the non-equilibrium parts by
-            # 1 - omega_1 = -0.994, so one-quantum tie flips are carried from
-            # step to step almost undamped (2.2% of the slots after 3 steps,
-            # against ~1% at nu = 0.01)
-            assert np.mean(diff > 0.5 * quantum) < 0.05, t
+            gf = g.export_f64().cpu().numpy()
+            diff = np.abs((gf - Wb) - (h.export_f64() - Wb))
+            assert (diff <= 1.001 * quantum).all(), (t, (diff / quantum).max())
+            flipped = np.mean(diff > 0.5 * quantum)
+            print(f"step {t}: {100 * flipped:.3f}% of the slots one quantum apart")
+            assert flipped < 0.005, t
+            h.import_f64(gf)  # the next step of both starts from the GPU's state
     assert np.isfinite(g.macros()[1].cpu().numpy()).all()
 
