@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define TNL_LBM_ABI_VERSION 3u
+#define TNL_LBM_ABI_VERSION 4u
 
 /* ---- status codes ------------------------------------------------------ */
 #define TNL_LBM_OK 0
@@ -205,6 +205,16 @@ typedef struct tnl_lbm_d3q27_data {
 	 * with a force field is stepped by the map kernel as all PERIODIC.  The
 	 * adjoint entry points do not take it (TNL_LBM_ERR_UNSUPPORTED).         */
 	const void* force_field;
+	/* Row lists of this map (no counterpart in the reference; NULL = none):
+	 * the output of tnl_lbm_classify_rows(SD, NULL, ...).  A step over the
+	 * whole box (offset 0, end = X, Y, Z) of the default cumulant in a
+	 * deviation storage form then runs each list through its own kernel:
+	 * rows of FLUID and PERIODIC sites only, rows that also hold WALL sites,
+	 * and the remaining rows through the kernel with every boundary code
+	 * (the F64 and F32 forms gain nothing from it and ignore the lists).  The
+	 * lists must describe the current map: classify again after changing it. */
+	const uint32_t* rows;
+	int64_t row_count[3];
 } tnl_lbm_d3q27_data;
 
 /* D3Q7 ADE data: ADE_Data (lbm_data.h:212-246) plus a source term that the
@@ -228,6 +238,11 @@ typedef struct tnl_lbm_d3q7_data {
 	/* deviation storage forms only: stored = (f_q - w_q*phi_ref)*store_scale */
 	double phi_ref;
 	double store_scale;
+	/* Row lists of the coupled step (no counterpart in the reference; NULL =
+	 * none): the output of tnl_lbm_classify_rows(nse, ade, ...) for this map
+	 * and the NSE lattice's, used by tnl_lbm_nse_ade_step over the whole box. */
+	const uint32_t* rows;
+	int64_t row_count[3];
 } tnl_lbm_d3q7_data;
 
 /* ======================================================================= */
@@ -242,6 +257,19 @@ const char* tnl_lbm_last_error(void);
  * launch of this process take the 64-bit form (same results, bit for bit),
  * on = 0 restores the default.  Returns the previous setting.              */
 int tnl_lbm_set_wide_addressing(int on);
+
+/* No counterpart in the reference: classification of the interior (x, z)
+ * rows for the row-list launches (the rows, row_count fields above).  A row
+ * is class 1 when every site of it is FLUID or PERIODIC (and, with ade, every
+ * D3Q7 code FLUID or PERIODIC), class 2 when WALL sites occur as well, class
+ * 0 otherwise; a NULL map counts as all PERIODIC.  rows: device array of at
+ * least X*Z entries (caller-owned), filled with the rows of class 0, then
+ * class 1, then class 2, each in increasing x*Z + z order, each entry
+ * x << 16 | z; row_count: host array, receives the three counts.  Needs
+ * X, Z < 65536.  Allocates temporary device memory and waits for the stream
+ * (a set-up call, not a step-path one).                                    */
+int tnl_lbm_classify_rows(const tnl_lbm_d3q27_data* nse, const tnl_lbm_d3q7_data* ade, uint32_t* rows,
+						  int64_t row_count[3], void* stream);
 
 /* ---- D3Q27 lattice update ----------------------------------------------
  * Replaces one launch of cudaLBMKernel<NSE>(SD, offset, end, distributed,

@@ -1,7 +1,9 @@
 """The 32-bit slot-offset form of the map, coupled, recording and FP64
 adjoint kernels (a per-direction base plus a 32-bit offset, taken where a
 population array holds at most 4 GiB) against the 64-bit form of the same
-kernels, forced by tnl_lbm_set_wide_addressing: the same bits."""
+kernels, forced by tnl_lbm_set_wide_addressing: the same bits.  The
+lattices step every row through the kernel with all boundary codes
+(row_lists=False), the kernels these forms were measured on."""
 import ctypes as C
 
 import numpy as np
@@ -51,7 +53,8 @@ def test_map_kernel_two_byte_forms(storage, pattern):
     X, Y, Z = 6, 70, 5
 
     def run():
-        g = D3Q27Lattice(X, Y, Z, storage=storage, pattern=pattern, nu=0.02, geo=duct(X, Y, Z), store_range=0.5)
+        g = D3Q27Lattice(X, Y, Z, storage=storage, pattern=pattern, nu=0.02, geo=duct(X, Y, Z), store_range=0.5,
+                         row_lists=False)
         g.set_inflow(0.03, 0.0, 0.0)
         g.set_equilibrium_field(*fields(X, Y, Z, 3))
         for t in range(4):
@@ -73,7 +76,7 @@ def test_coupled_kernel_two_byte_forms(storage, pattern):
     ageo[:, :, 0] = ageo[:, :, -1] = ADE_GEO.WALL
 
     def run():
-        g = D3Q27Lattice(X, Y, Z, storage=storage, pattern=pattern, nu=0.02, geo=geo, store_range=0.5)
+        g = D3Q27Lattice(X, Y, Z, storage=storage, pattern=pattern, nu=0.02, geo=geo, store_range=0.5, row_lists=False)
         a = D3Q7Lattice(g, collision="CLBM", diffusivity=0.0125, geo=ageo, phi_ref=0.0, store_scale=1000.0)
         g.set_equilibrium_field(*fields(X, Y, Z, 4))
         a.set_equilibrium(torch.tensor(np.random.default_rng(5).random((X, Y, Z)) * 1e-3))
@@ -92,7 +95,7 @@ def test_recording_step(storage, tape_f64):
     X, Y, Z = 6, 70, 5
 
     def run():
-        g = D3Q27Lattice(X, Y, Z, storage=storage, pattern="AB", nu=0.02, geo=duct(X, Y, Z))
+        g = D3Q27Lattice(X, Y, Z, storage=storage, pattern="AB", nu=0.02, geo=duct(X, Y, Z), row_lists=False)
         g.set_inflow(0.03, 0.0, 0.0)
         g.set_equilibrium_field(*fields(X, Y, Z, 6))
         tape = torch.zeros(tape_elements(g.d.XYZ), dtype=torch.float64 if tape_f64 else torch.float32, device="cuda")

@@ -75,6 +75,16 @@ struct Ade7
 		f[3] = eq<3>(phi, vx, vy, vz); f[4] = eq<4>(phi, vx, vy, vz); f[5] = eq<5>(phi, vx, vy, vz);
 		f[6] = eq<6>(phi, vx, vy, vz);
 	}
+	// full-way bounce-back of the six moving populations (d3q7/bc.h, GEO_WALL)
+	static LBM_DEV void bounce_back(real* f)
+	{
+#pragma unroll
+		for (int q = 1; q < 7; q += 2) {
+			const real t = f[q];
+			f[q] = f[q + 1];
+			f[q + 1] = t;
+		}
+	}
 	static LBM_DEV real density(const real* f)
 	{
 		real phi = 0;

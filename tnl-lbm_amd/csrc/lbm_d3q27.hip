@@ -222,6 +222,52 @@ int tnl_lbm_set_wide_addressing(int on)
 	return g_wide_addressing.exchange(on ? 1 : 0);
 }
 
+int tnl_lbm_classify_rows(const tnl_lbm_d3q27_data* nse, const tnl_lbm_d3q7_data* ade, uint32_t* rows,
+						  int64_t row_count[3], void* stream)
+{
+	if (! nse || ! rows || ! row_count)
+		return fail(TNL_LBM_ERR_NULL, "null data, rows or row_count");
+	if (nse->struct_size != sizeof(tnl_lbm_d3q27_data) || nse->version != TNL_LBM_ABI_VERSION)
+		return fail(TNL_LBM_ERR_VERSION, "tnl_lbm_d3q27_data struct_size/version mismatch");
+	if (ade) {
+		if (ade->struct_size != sizeof(tnl_lbm_d3q7_data) || ade->version != TNL_LBM_ABI_VERSION)
+			return fail(TNL_LBM_ERR_VERSION, "tnl_lbm_d3q7_data struct_size/version mismatch");
+		if (ade->X != nse->X || ade->Y != nse->Y || ade->Z != nse->Z || ade->ox != nse->ox || ade->oy != nse->oy
+			|| ade->oz != nse->oz || ade->XYZ != nse->XYZ)
+			return fail(TNL_LBM_ERR_DIMS, "NSE and ADE lattices must have the same geometry");
+	}
+	if (nse->X < 0 || nse->Y < 0 || nse->Z < 0 || nse->X >= 65536 || nse->Z >= 65536)
+		return fail(TNL_LBM_ERR_DIMS, "row lists need 0 <= X, Z < 65536");
+	for (int c = 0; c < 3; c++)
+		row_count[c] = 0;
+	const int64_t n = nse->X * nse->Z;
+	if (n == 0)
+		return TNL_LBM_OK;
+	hipStream_t st = (hipStream_t) stream;
+	void* buf = nullptr;
+	const size_t cls_bytes = ((size_t) n + 255) / 256 * 256;
+	if (int rc = hip_check(hipMalloc(&buf, cls_bytes + 3 * sizeof(uint32_t)), "classify scratch"))
+		return rc;
+	uint8_t* cls = (uint8_t*) buf;
+	uint32_t* counts = (uint32_t*) ((char*) buf + cls_bytes);
+	const int64_t Ys = nse->Y + 2 * nse->oy, Zs = nse->Z + 2 * nse->oz;
+	hipLaunchKernelGGL(k_classify_rows, dim3((unsigned) ((n * 64 + 255) / 256)), dim3(256), 0, st, nse->map,
+					   ade ? ade->map : nullptr, nse->X, nse->Y, nse->Z, nse->ox, nse->oy, nse->oz, Ys, Zs, cls);
+	hipLaunchKernelGGL(k_row_lists, dim3(1), dim3(1024), 0, st, cls, (uint32_t) n, (uint32_t) nse->Z, rows, counts);
+	uint32_t host[3] = {0, 0, 0};
+	int rc = hip_check(hipGetLastError(), "classify launch");
+	if (! rc)
+		rc = hip_check(hipMemcpyAsync(host, counts, sizeof host, hipMemcpyDeviceToHost, st), "classify counts");
+	if (! rc)
+		rc = hip_check(hipStreamSynchronize(st), "classify");
+	const int frc = hip_check(hipFree(buf), "classify scratch free");
+	if (rc)
+		return rc;
+	for (int c = 0; c < 3; c++)
+		row_count[c] = host[c];
+	return frc;
+}
+
 static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t offset[3],
 						  const int64_t end[3], uint32_t flags, StepArgs& A, unsigned& threads, unsigned& grid);
 
@@ -257,6 +303,23 @@ int tnl_lbm_d3q27_stream_only(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_dat
 		A.cls_inv[n] = A.cls_sc[n] > 0 ? (float) (1.0 / ((double) A.cls_sc[n] * 32767.0)) : 0.0f;
 	hipStream_t st = (hipStream_t) stream;
 	STORAGE_SWITCH(cfg->storage, (launch_periodic<ST, COLL_STREAM_ONLY>(A, threads, grid, st)));
+}
+
+// The three row lists of a full-box launch, checked against the box
+static int set_lists(StepArgs& A, int64_t X, int64_t Z, const uint32_t* rows, const int64_t count[3])
+{
+	int64_t sum = 0;
+	for (int c = 0; c < 3; c++) {
+		if (count[c] < 0)
+			return fail(TNL_LBM_ERR_ARG, "negative row count");
+		sum += count[c];
+	}
+	if (sum > X * Z || X >= 65536 || Z >= 65536 || (uint64_t) sum * (uint64_t) A.ny >= (1ull << 31))
+		return fail(TNL_LBM_ERR_DIMS, "row lists do not fit the box");
+	A.lists = rows;
+	for (int c = 0; c < 3; c++)
+		A.list_count[c] = count[c];
+	return TNL_LBM_OK;
 }
 
 static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* SD, const int64_t offset[3],
@@ -328,6 +391,12 @@ static int make_step_args(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* S
 		A.prof[k] = SD->inflow_profile[k];
 	A.prof_f32 = SD->inflow_profile_real == TNL_LBM_REAL_F32;
 	A.force = SD->force_field;
+	// the row lists apply to a launch over the whole box
+	const bool full = offset[0] == 0 && offset[1] == 0 && offset[2] == 0 && end[0] == SD->X && end[1] == SD->Y
+					  && end[2] == SD->Z;
+	if (full && SD->rows)
+		if (int rc = set_lists(A, SD->X, SD->Z, SD->rows, SD->row_count))
+			return rc;
 	grid = padded;
 	return TNL_LBM_OK;
 }
@@ -380,25 +449,48 @@ int tnl_lbm_nse_ade_step(const tnl_lbm_config* cfg, const tnl_lbm_d3q27_data* ns
 	B.collision = ade->collision;
 	B.even = ade->even_iter ? 1 : 0;
 	hipStream_t st = (hipStream_t) stream;
-	linearize(A, threads, grid);
+	// the coupled step takes the lists classified from both maps (the ADE
+	// data's), never the NSE lattice's own
+	A.lists = nullptr;
+	const bool full = offset[0] == 0 && offset[1] == 0 && offset[2] == 0 && end[0] == nse->X && end[1] == nse->Y
+					  && end[2] == nse->Z;
+	if (full && ade->rows)
+		if (int rc = set_lists(A, nse->X, nse->Z, ade->rows, ade->row_count))
+			return rc;
 	auto go = [&](auto stc) {
 		constexpr int ST = decltype(stc)::value;
 		// 2-byte forms: 32-bit slot offsets where each population array (D3Q27
 		// and D3Q7 share XYZ) holds at most 4 GiB, as in the map kernel
 		constexpr bool two = sizeof(typename Storage<ST>::elem) == 2;
-		if (two && narrow_slots(A.XYZ, 2)) {
-			if constexpr (two) {
-				if (cfg->pattern == TNL_LBM_PATTERN_AA)
-					hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA, true>), dim3(grid), dim3(threads), 0, st, A, B);
-				else
-					hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, true>), dim3(grid), dim3(threads), 0, st, A, B);
+		const bool o32 = two && narrow_slots(A.XYZ, 2);
+		auto one = [&](auto lc, const StepArgs& C, unsigned t, unsigned g) {
+			constexpr int LEAN = decltype(lc)::value;
+			if (o32) {
+				if constexpr (two) {
+					if (cfg->pattern == TNL_LBM_PATTERN_AA)
+						hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA, true, LEAN>), dim3(g), dim3(t), 0, st, C, B);
+					else
+						hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, true, LEAN>), dim3(g), dim3(t), 0, st, C, B);
+				}
 			}
-		}
-		else if (cfg->pattern == TNL_LBM_PATTERN_AA)
-			hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, A, B);
-		else
-			hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, A, B);
-		return hip_check(hipGetLastError(), "nse_ade launch");
+			else if (cfg->pattern == TNL_LBM_PATTERN_AA)
+				hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AA, false, LEAN>), dim3(g), dim3(t), 0, st, C, B);
+			else
+				hipLaunchKernelGGL((k_nse_ade<ST, TNL_LBM_COLL_CUM, TNL_LBM_PATTERN_AB, false, LEAN>), dim3(g), dim3(t), 0, st, C, B);
+			return hip_check(hipGetLastError(), "nse_ade launch");
+		};
+		if (A.lists)
+			return for_row_lists(A, [&](const StepArgs& C, unsigned t, unsigned g, int lean) {
+				if (lean == 1)
+					return one(std::integral_constant<int, 1>{}, C, t, g);
+				if (lean == 2)
+					return one(std::integral_constant<int, 2>{}, C, t, g);
+				return one(std::integral_constant<int, 3>{}, C, t, g);
+			});
+		StepArgs C = A;
+		unsigned t = threads, g = grid;
+		linearize(C, t, g);
+		return one(std::integral_constant<int, 0>{}, C, t, g);
 	};
 	STORAGE_SWITCH(cfg->storage, go(std::integral_constant<int, ST>{}));
 }

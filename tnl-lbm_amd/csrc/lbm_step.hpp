@@ -64,6 +64,14 @@ struct StepArgs
 	int32_t prof_f32;
 	const void* force;  // per-site force: 3 planes of macro elements (macro_f64), or null
 	void* tape;  // moment recording of the A-B step (10 planes of XYZ), or null
+	// a launch over one row list (tnl_lbm_classify_rows): the listed (x, z)
+	// rows of the box, each entry x << 16 | z; null = the box itself
+	const uint32_t* rows;
+	uint32_t nrows;
+	// host side: the three lists of a full-box step (classes 0, 1, 2 one
+	// after another), or null
+	const uint32_t* lists;
+	int64_t list_count[3];
 };
 
 constexpr int NXCD = 8;
@@ -101,13 +109,39 @@ LBM_DEV double inflow_component(const StepArgs& A, int k, int64_t i)
 // logical block -> (x, z, y segment) with XCD-aware remapping
 // LIN = false: whole rows only (x and z stay uniform per block, as the
 // periodic kernel's scalar addressing needs)
-template <bool LIN = true>
+// LIST: a launch over a row list (A.rows); a compile-time choice, since
+// coordinates that may come from either source are per-lane values in both
+// (the coupled kernel's registers rose from 109 to 133 with a run-time one)
+template <bool LIN = true, bool LIST = false>
 LBM_DEV bool block_coords(const StepArgs& A, int64_t& x, int64_t& z, int64_t& y)
 {
 	const uint32_t b = blockIdx.x;
 	const uint32_t L = (b % NXCD) * A.per_xcd + b / NXCD;
 	if (L >= A.nblocks)
 		return false;
+	if constexpr (LIST) {
+		// a row list: whole rows per block, or the listed rows' sites one
+		// after another (linear mapping)
+		uint32_t k, yy;
+		if (LIN && A.lin) {
+			const uint32_t i = L * blockDim.x + threadIdx.x;
+			k = i / (uint32_t) A.ny;
+			if (k >= A.nrows)
+				return false;
+			yy = i % (uint32_t) A.ny;
+		}
+		else {
+			k = L / A.nyb;
+			yy = (L % A.nyb) * blockDim.x + threadIdx.x;
+			if (yy >= (uint32_t) A.ny)
+				return false;
+		}
+		const uint32_t r = A.rows[k];
+		x = (int64_t) (r >> 16);
+		z = (int64_t) (r & 0xffffu);
+		y = (int64_t) yy;
+		return x < A.X && z < A.Z;
+	}
 	if (LIN && A.lin) {
 		// consecutive lanes walk the y-z plane of one x, row after row
 		const uint32_t i = (L % A.nyb) * blockDim.x + threadIdx.x;
@@ -680,7 +714,15 @@ LBM_DEV E* slot0(E* base, int q, const StepArgs& A, int64_t s)
 // byte offset. The site index is then formed in 32-bit arithmetic and each
 // access is one scalar-base load or store, in place of a 64-bit address per
 // lane and access.
-template <int ST, int COLL, int PATTERN, int TAPE = 0, bool O32 = false>
+//
+// LEAN: the sites of the launch carry only FLUID or PERIODIC codes (1), or
+// those and WALL (2), as the row lists of tnl_lbm_classify_rows guarantee;
+// every other boundary path is left out of the kernel.  (The kernels take
+// LEAN 3 for the general paths over a row list: General sees 0.)  A kernel's register
+// allocation is the maximum over all of its paths, so the rarely taken ones
+// set it for every site: the F16 A-B map kernel needs 97 VGPRs with all of
+// them, 49 with FLUID and PERIODIC alone.
+template <int ST, int COLL, int PATTERN, int TAPE = 0, bool O32 = false, int LEAN = 0>
 struct General
 {
 	using S = Storage<ST>;
@@ -763,6 +805,19 @@ struct General
 		site(A, x, y, z, vx, vy, vz, NX, NY, NZ);
 	}
 
+	// full-way bounce-back (bc.h:201-221)
+	static LBM_DEV void bounce_back(real* f, real& rho, real& vx, real& vy, real& vz)
+	{
+		rho = 1;
+		vx = vy = vz = 0;
+#pragma unroll
+		for (int q = 1; q < 27; q += 2) {
+			const real t = f[q];
+			f[q] = f[q + 1];
+			f[q + 1] = t;
+		}
+	}
+
 	// one site; returns the pre-collision velocity and the neighbour indices
 	// (the coupled kernel drives the ADE step with them, kernels.h:152-176)
 	static LBM_DEV void site(const StepArgs& A, int64_t x, int64_t y, int64_t z, real& ovx, real& ovy, real& ovz, Nb& NX,
@@ -781,12 +836,12 @@ struct General
 		real rho = 1, vx = 0, vy = 0, vz = 0;
 		const elem* __restrict__ cur = (const elem*) A.df0;
 
-		if (gi != GEO_NOTHING) {
+		if (LEAN || gi != GEO_NOTHING) {
 			// pull location (bc.h:72-74); OUTFLOW_RIGHT pulls as if located at x-1
 			Nb PX = NX;
-			if (gi == GEO_OUTFLOW_RIGHT)
+			if (! LEAN && gi == GEO_OUTFLOW_RIGHT)
 				PX.p = PX.o = NX.m;
-			if (gi == GEO_OUTFLOW_RIGHT_INTERP) {
+			if (! LEAN && gi == GEO_OUTFLOW_RIGHT_INTERP) {
 				// streamingInterpRight (streaming_AB.h:446-478)
 				const real cs = (real) 0.5773502691896257;
 #pragma unroll
@@ -827,7 +882,13 @@ struct General
 					tm[k] = (TT) m[k];
 			}
 			const real rest = S::dev ? (real) 1 : (real) 0;
-			switch (gi) {
+			if constexpr (LEAN != 0) {
+				if (LEAN == 2 && gi == GEO_WALL)
+					bounce_back(f, rho, vx, vy, vz);
+				else if (! FAST)
+					P::density_velocity(f, Fx, Fy, Fz, rho, vx, vy, vz);
+			}
+			else switch (gi) {
 				case GEO_INFLOW:
 					inflow(A, y, z, vx, vy, vz);
 					rho = 1;
@@ -905,15 +966,7 @@ struct General
 					rho = 1;
 					break;
 				case GEO_WALL:
-					// full-way bounce-back (bc.h:201-221)
-					rho = 1;
-					vx = vy = vz = 0;
-#pragma unroll
-					for (int q = 1; q < 27; q += 2) {
-						const real t = f[q];
-						f[q] = f[q + 1];
-						f[q + 1] = t;
-					}
+					bounce_back(f, rho, vx, vy, vz);
 					break;
 				case GEO_SYM_TOP:
 				case GEO_SYM_BOTTOM:
@@ -950,20 +1003,21 @@ struct General
 					break;
 			}
 			// doCollision (bc.h:406-412)
-			const bool do_coll = gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_RIGHT_INTERP
-							  || gi == GEO_INFLOW_LEFT || gi == GEO_INFLOW_BB_LEFT;
+			const bool do_coll = LEAN ? (LEAN == 1 || gi != GEO_WALL)
+									  : (gi == GEO_FLUID || gi == GEO_PERIODIC || gi == GEO_OUTFLOW_RIGHT || gi == GEO_OUTFLOW_RIGHT_INTERP
+										 || gi == GEO_INFLOW_LEFT || gi == GEO_INFLOW_BB_LEFT);
 			if constexpr (FAST) {
 				// FP32 forms, default cumulant: the raw-moment formulation (as the periodic kernel)
 				// wherever rho and u are the site's own; the boundary codes that replace them keep
 				// the reference chain
-				if (gi == GEO_FLUID || gi == GEO_PERIODIC) {
+				if (LEAN ? do_coll : (gi == GEO_FLUID || gi == GEO_PERIODIC)) {
 					using FC = FastCum<S::dev>;
 					FC::collide(f, Fx, Fy, Fz, (real) A.omega1, true, rho, vx, vy, vz);
 #pragma unroll
 					for (int q = 0; q < 27; q++)
 						f[q] *= FC::unscale(q);
 				}
-				else if (do_coll)
+				else if (! LEAN && do_coll)
 					collide<ST, COLL>(f, rho, vx, vy, vz, A, Fx, Fy, Fz);
 			}
 			else if (do_coll)
@@ -1007,14 +1061,16 @@ struct General
 // The FP64 default cumulant needs ~170 VGPRs here: asking for 3 waves per SIMD
 // (<= 168) costs at most 2 spilled registers; the alternative operators keep
 // their registers.
-template <int ST, int COLL, int PATTERN, int TAPE = 0, bool O32 = false>
+// LEAN 0: the box (or sub-box) of the launch, every code; 1, 2: a row list
+// of that lean level; 3: a row list through the general paths
+template <int ST, int COLL, int PATTERN, int TAPE = 0, bool O32 = false, int LEAN = 0>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(COLL == TNL_LBM_COLL_CUM ? 3 : 1, 8))) k_general(const StepArgs A)
 {
 	int64_t x, z, y;
-	if (! block_coords(A, x, z, y))
+	if (! block_coords<true, LEAN != 0>(A, x, z, y))
 		return;
-	General<ST, COLL, PATTERN, TAPE, O32>::site(A, x, y, z);
+	General<ST, COLL, PATTERN, TAPE, O32, LEAN == 3 ? 0 : LEAN>::site(A, x, y, z);
 }
 
 // ===========================================================================
@@ -1032,7 +1088,7 @@ struct AdeIn
 	typename Storage<ST>::real D, S;
 };
 
-template <int ST, int PATTERN, bool O32 = false>
+template <int ST, int PATTERN, bool O32 = false, int LEAN = 0>
 LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb& NX, const Nb& NY, const Nb& NZ,
 					  AdeIn<ST>& in)
 {
@@ -1049,10 +1105,10 @@ LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb&
 	// D3Q7_MACRO_Default::copyQuantities (d3q7/macro.h:38-42)
 	in.D = B.diff ? macro_at(B.diff) : (real) B.D;
 	in.S = B.source ? macro_at(B.source) : (real) 0;
-	if (gi != A_NOTHING) {
+	if (LEAN || gi != A_NOTHING) {
 		const elem* cur = (const elem*) B.df0;
 		Nb PX = NX;
-		if (gi == A_OUTFLOW_RIGHT)
+		if (! LEAN && gi == A_OUTFLOW_RIGHT)
 			PX.p = PX.o = NX.m;
 		// nontemporal hints as in the D3Q27 update: own slots and c_y = 0 pulls
 		constexpr bool NT = (LBM_NT_GENERAL >> ST) & 1;
@@ -1075,7 +1131,8 @@ LBM_DEV void ade_load(const StepArgs& A, const AdeArgs& B, int64_t s0, const Nb&
 	}
 }
 
-template <int ST, int PATTERN, bool O32 = false>
+// LEAN as in General: 1 = A_FLUID and A_PERIODIC sites only, 2 = those and A_WALL
+template <int ST, int PATTERN, bool O32 = false, int LEAN = 0>
 LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename Storage<ST>::real vx,
 					  typename Storage<ST>::real vy, typename Storage<ST>::real vz, const Nb& NX, const Nb& NY, const Nb& NZ,
 					  const AdeIn<ST>& in)
@@ -1088,24 +1145,24 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 	const real pr = (real) B.phi_ref, sc = (real) B.scale;
 	const real D = in.D;
 	real phi = 0;
-	if (gi != A_NOTHING) {
+	if (LEAN || gi != A_NOTHING) {
 		real f[7];
 #pragma unroll
 		for (int q = 0; q < 7; q++)
 			f[q] = in.f[q];
-		switch (gi) {
+		if constexpr (LEAN != 0) {
+			phi = K::density(f);
+			if (LEAN == 2 && gi == A_WALL)
+				K::bounce_back(f);
+		}
+		else switch (gi) {
 			case A_INFLOW:
 				phi = (real) B.inflow_phi;
 				K::set_eq(f, phi, vx, vy, vz);
 				break;
 			case A_WALL:
 				phi = K::density(f);
-#pragma unroll
-				for (int q = 1; q < 7; q += 2) {
-					const real t = f[q];
-					f[q] = f[q + 1];
-					f[q + 1] = t;
-				}
+				K::bounce_back(f);
 				break;
 			case A_SYM_TOP: f[zzm] = f[zzp]; phi = K::density(f); break;
 			case A_SYM_BOTTOM: f[zzp] = f[zzm]; phi = K::density(f); break;
@@ -1115,7 +1172,7 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 			case A_SYM_FRONT: f[zmz] = f[zpz]; phi = K::density(f); break;
 			default: phi = K::density(f); break;
 		}
-		if (gi == A_FLUID || gi == A_PERIODIC || gi == A_SOLID || gi == A_OUTFLOW_RIGHT) {
+		if (LEAN ? (LEAN == 1 || gi != A_WALL) : (gi == A_FLUID || gi == A_PERIODIC || gi == A_SOLID || gi == A_OUTFLOW_RIGHT)) {
 			K::collide(B.collision, f, phi, vx, vy, vz, D);
 			if (B.source) {
 #pragma unroll
@@ -1152,11 +1209,13 @@ LBM_DEV void ade_site(const StepArgs& A, const AdeArgs& B, int64_t s0, typename 
 	}
 }
 
-template <int ST, int COLL, int PATTERN, bool O32 = false>
+// LEAN as for k_general
+template <int ST, int COLL, int PATTERN, bool O32 = false, int LEAN = 0>
 __global__ void __launch_bounds__(256) k_nse_ade(const StepArgs A, const AdeArgs B)
 {
+	constexpr int LV = LEAN == 3 ? 0 : LEAN;
 	int64_t x, z, y;
-	if (! block_coords(A, x, z, y))
+	if (! block_coords<true, LEAN != 0>(A, x, z, y))
 		return;
 	constexpr bool AA = PATTERN == TNL_LBM_PATTERN_AA;
 	const int64_t s0 = ((x + A.ox) * A.Zs + (z + A.oz)) * A.Ys + (y + A.oy);
@@ -1166,11 +1225,11 @@ __global__ void __launch_bounds__(256) k_nse_ade(const StepArgs A, const AdeArgs
 	const Nb NY = neighbours<AA>(y, A.Y, A.oy, A.dist_y, per);
 	const Nb NZ = neighbours<AA>(z, A.Z, A.oz, A.dist_z, per);
 	AdeIn<ST> in;
-	ade_load<ST, PATTERN, O32>(A, B, s0, NX, NY, NZ, in);
+	ade_load<ST, PATTERN, O32, LV>(A, B, s0, NX, NY, NZ, in);
 	typename Storage<ST>::real vx, vy, vz;
 	Nb GX, GY, GZ;
-	General<ST, COLL, PATTERN, 0, O32>::site(A, x, y, z, vx, vy, vz, GX, GY, GZ);
-	ade_site<ST, PATTERN, O32>(A, B, s0, vx, vy, vz, NX, NY, NZ, in);
+	General<ST, COLL, PATTERN, 0, O32, LV>::site(A, x, y, z, vx, vy, vz, GX, GY, GZ);
+	ade_site<ST, PATTERN, O32, LV>(A, B, s0, vx, vy, vz, NX, NY, NZ, in);
 }
 
 template <int ST>
@@ -1203,6 +1262,105 @@ __global__ void k_ade_export(const void* df, double* out, int64_t XYZ, int64_t n
 	for (int q = 0; q < 7; q++)
 		out[q * XYZ + s] = (double) K::ld((const typename K::elem*) df + q * XYZ + s, q, (typename K::real) pr,
 										  (typename K::real) 1 / (typename K::real) sc);
+}
+
+// ===========================================================================
+// Row classification for the row-list launches (tnl_lbm_classify_rows)
+// ===========================================================================
+// One wave per interior (x, z) row, row i = x * Z + z: class 1 when every
+// code of the row is FLUID or PERIODIC (D3Q27 and, with amap, D3Q7), 2 when
+// WALL sites occur as well, 0 otherwise.  A null map is all PERIODIC.
+static __global__ void __launch_bounds__(256) k_classify_rows(const int16_t* map, const int16_t* amap, int64_t X, int64_t Y,
+													   int64_t Z, int64_t ox, int64_t oy, int64_t oz, int64_t Ys, int64_t Zs,
+													   uint8_t* cls)
+{
+	const int64_t i = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // uniform per wave
+	if (i >= X * Z)
+		return;
+	const int64_t x = i / Z, z = i % Z;
+	const int64_t base = ((x + ox) * Zs + (z + oz)) * Ys + oy;
+	bool other = false, wall = false;
+	for (int64_t y = threadIdx.x & 63; y < Y; y += 64) {
+		if (map) {
+			const int16_t c = map[base + y];
+			wall |= c == GEO_WALL;
+			other |= ! (c == GEO_FLUID || c == GEO_PERIODIC || c == GEO_WALL);
+		}
+		if (amap) {
+			const int16_t c = amap[base + y];
+			wall |= c == A_WALL;
+			other |= ! (c == A_FLUID || c == A_PERIODIC || c == A_WALL);
+		}
+	}
+	const uint64_t o = __ballot(other), w = __ballot(wall);
+	if ((threadIdx.x & 63) == 0)
+		cls[i] = o ? 0 : (w ? 2 : 1);
+}
+
+// The three row lists from the classes, in one workgroup so that each list
+// keeps increasing row order: counts first, then every chunk of 1024 rows
+// places its rows of each class after those of the earlier chunks.
+static __global__ void __launch_bounds__(1024) k_row_lists(const uint8_t* cls, uint32_t n, uint32_t Z, uint32_t* rows,
+													uint32_t* counts)
+{
+	__shared__ uint32_t part[16][3];
+	__shared__ uint32_t off[3];
+	const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+	uint32_t c3[3] = {0, 0, 0};
+	for (uint32_t i = t; i < n; i += 1024) {
+		const uint8_t c = cls[i];
+		c3[0] += c == 0;
+		c3[1] += c == 1;
+		c3[2] += c == 2;
+	}
+	for (int k = 0; k < 3; k++) {
+		uint32_t v = c3[k];
+		for (int d = 32; d >= 1; d >>= 1)
+			v += __shfl_xor(v, d);
+		if (lane == 0)
+			part[wave][k] = v;
+	}
+	__syncthreads();
+	if (t == 0) {
+		uint32_t tot[3] = {0, 0, 0};
+		for (int w = 0; w < 16; w++)
+			for (int k = 0; k < 3; k++)
+				tot[k] += part[w][k];
+		for (int k = 0; k < 3; k++)
+			counts[k] = tot[k];
+		off[0] = 0;
+		off[1] = tot[0];
+		off[2] = tot[0] + tot[1];
+	}
+	__syncthreads();
+	const uint64_t below = (lane == 0) ? 0 : (~0ull >> (64 - lane));
+	for (uint32_t c0 = 0; c0 < n; c0 += 1024) {
+		const uint32_t i = c0 + t;
+		const int c = i < n ? (int) cls[i] : -1;
+		uint64_t m[3];
+		for (int k = 0; k < 3; k++) {
+			m[k] = __ballot(c == k);
+			if (lane == 0)
+				part[wave][k] = (uint32_t) __popcll(m[k]);
+		}
+		__syncthreads();
+		if (c >= 0) {
+			uint32_t pos = off[c];
+			for (uint32_t w = 0; w < wave; w++)
+				pos += part[w][c];
+			const uint64_t mc = c == 0 ? m[0] : (c == 1 ? m[1] : m[2]);
+			pos += (uint32_t) __popcll(mc & below);
+			rows[pos] = ((i / Z) << 16) | (i % Z);
+		}
+		__syncthreads();
+		if (t < 3) {
+			uint32_t add = 0;
+			for (int w = 0; w < 16; w++)
+				add += part[w][t];
+			off[t] += add;
+		}
+		__syncthreads();
+	}
 }
 
 // ===========================================================================
@@ -1436,6 +1594,51 @@ inline void linearize(StepArgs& A, unsigned& threads, unsigned& grid)
 	grid = padded;
 }
 
+// Launch shape over one row list of a full-box launch: whole rows per block
+// where the row length is a multiple of the block, else the listed rows'
+// sites one after another, as linearize does for the box.
+inline void list_shape(StepArgs& A, const uint32_t* rows, uint32_t nrows, unsigned& threads, unsigned& grid)
+{
+	A.rows = rows;
+	A.nrows = nrows;
+	threads = A.ny > 128 ? 256 : (A.ny > 64 ? 128 : 64);
+	if (A.ny % (int32_t) threads == 0) {
+		A.lin = 0;
+		A.nyb = (uint32_t) A.ny / threads;
+		A.nblocks = A.nyb * nrows;
+	}
+	else {
+		threads = 256;
+		A.lin = 1;
+		A.nyb = 1;
+		A.nblocks = (uint32_t) (((uint64_t) nrows * (uint64_t) A.ny + threads - 1) / threads);
+	}
+	const uint32_t padded = (A.nblocks + NXCD - 1) / NXCD * NXCD;
+	A.per_xcd = padded / NXCD;
+	grid = padded;
+}
+
+// The row lists of a full-box step: one launch per non-empty list, the lean
+// kernels (LEAN 1, 2) for classes 1 and 2 and the general paths over a list
+// (LEAN 3) for class 0.  go(B, threads, grid, lean) makes one launch.
+template <typename Go>
+int for_row_lists(const StepArgs& A, Go&& go)
+{
+	int64_t off = 0;
+	for (int c = 0; c < 3; c++) {
+		const int64_t n = A.list_count[c];
+		if (n > 0) {
+			StepArgs B = A;
+			unsigned threads = 0, grid = 0;
+			list_shape(B, A.lists + off, (uint32_t) n, threads, grid);
+			if (int rc = go(B, threads, grid, c == 0 ? 3 : c))
+				return rc;
+		}
+		off += n;
+	}
+	return 0;
+}
+
 // Launch shape of k_aa_even_pair (one lane per two sites of a row); false
 // where the pairs would not be 4-byte aligned.
 inline bool pair_shape(const StepArgs& A, StepArgs& B, unsigned& threads, unsigned& grid)
@@ -1484,34 +1687,56 @@ int launch_periodic(const StepArgs& A, unsigned threads, unsigned grid, hipStrea
 	return hip_check(hipGetLastError(), "step launch");
 }
 
+// One launch of the map kernel at lean level LEAN (k_general)
+template <int ST, int COLL, int LEAN>
+void launch_general(int pattern, const StepArgs& B, unsigned threads, unsigned grid, hipStream_t st, bool o32)
+{
+	if (o32) {
+		if constexpr (sizeof(typename Storage<ST>::elem) == 2) {
+			if (pattern == TNL_LBM_PATTERN_AA)
+				hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA, 0, true, LEAN>), dim3(grid), dim3(threads), 0, st, B);
+			else
+				hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB, 0, true, LEAN>), dim3(grid), dim3(threads), 0, st, B);
+		}
+	}
+	else if (pattern == TNL_LBM_PATTERN_AA)
+		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA, 0, false, LEAN>), dim3(grid), dim3(threads), 0, st, B);
+	else
+		hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB, 0, false, LEAN>), dim3(grid), dim3(threads), 0, st, B);
+}
+
 template <int ST, int COLL>
 int launch_step(const tnl_lbm_config* cfg, const StepArgs& A, unsigned threads, unsigned grid, hipStream_t st)
 {
 	// the map-free periodic kernels take the homogeneous force only
 	if (! A.map && ! A.force && cfg->pattern == TNL_LBM_PATTERN_AA)
 		return launch_periodic<ST, COLL>(A, threads, grid, st);
-	else {
-		StepArgs B = A;
-		linearize(B, threads, grid);
-		// 2-byte forms: 32-bit slot offsets where a population array is under
-		// 4 GiB (F16 384^3 channel: A-A 1821 -> 1430 us, 139 -> 106 VGPRs;
-		// A-B -2%; neutral for the 4- and 8-byte forms, which keep 64-bit)
-		bool o32 = false;
-		if constexpr (sizeof(typename Storage<ST>::elem) == 2)
-			o32 = narrow_slots(A.XYZ, sizeof(typename Storage<ST>::elem));
-		if (o32) {
-			if constexpr (sizeof(typename Storage<ST>::elem) == 2) {
-				if (cfg->pattern == TNL_LBM_PATTERN_AA)
-					hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA, 0, true>), dim3(grid), dim3(threads), 0, st, B);
+	// 2-byte forms: 32-bit slot offsets where a population array is under
+	// 4 GiB (F16 384^3 channel: A-A 1821 -> 1430 us, 139 -> 106 VGPRs;
+	// A-B -2%; neutral for the 4- and 8-byte forms, which keep 64-bit)
+	bool o32 = false;
+	if constexpr (sizeof(typename Storage<ST>::elem) == 2)
+		o32 = narrow_slots(A.XYZ, sizeof(typename Storage<ST>::elem));
+	// row lists for the deviation forms, whose map kernel is bound by its
+	// registers and instruction issue (384^3 with z walls, lean against
+	// general: I16 -22%, F16 A-B -8%, BF16 -6%, F32_DEV -2%); the F64 and F32
+	// map kernels gain nothing, and a class-0 list costs them an extra launch
+	// (C4 duct +4 to +6%)
+	if constexpr (COLL == TNL_LBM_COLL_CUM && Storage<ST>::dev) {
+		if (A.lists)
+			return for_row_lists(A, [&](const StepArgs& B, unsigned t, unsigned g, int lean) {
+				if (lean == 1)
+					launch_general<ST, COLL, 1>(cfg->pattern, B, t, g, st, o32);
+				else if (lean == 2)
+					launch_general<ST, COLL, 2>(cfg->pattern, B, t, g, st, o32);
 				else
-					hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB, 0, true>), dim3(grid), dim3(threads), 0, st, B);
-			}
-		}
-		else if (cfg->pattern == TNL_LBM_PATTERN_AA)
-			hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AA>), dim3(grid), dim3(threads), 0, st, B);
-		else
-			hipLaunchKernelGGL((k_general<ST, COLL, TNL_LBM_PATTERN_AB>), dim3(grid), dim3(threads), 0, st, B);
+					launch_general<ST, COLL, 3>(cfg->pattern, B, t, g, st, o32);
+				return hip_check(hipGetLastError(), "step launch");
+			});
 	}
+	StepArgs B = A;
+	linearize(B, threads, grid);
+	launch_general<ST, COLL, 0>(cfg->pattern, B, threads, grid, st, o32);
 	return hip_check(hipGetLastError(), "step launch");
 }
 

@@ -10,7 +10,7 @@ import ctypes as C
 import os
 from pathlib import Path
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # status codes
 OK = 0
@@ -127,7 +127,8 @@ class D3Q27Data(C.Structure):
                 ("df", C.c_void_p * 2), ("macro", C.c_void_p), ("map", C.c_void_p),
                 ("inflow_vx", C.c_double), ("inflow_vy", C.c_double), ("inflow_vz", C.c_double),
                 ("inflow_profile", C.c_void_p * 3), ("store_range", C.c_double),
-                ("store_range_class", C.c_double * 4), ("force_field", C.c_void_p)]
+                ("store_range_class", C.c_double * 4), ("force_field", C.c_void_p),
+                ("rows", C.c_void_p), ("row_count", C.c_int64 * 3)]
 
     @classmethod
     def make(cls, X, Y, Z, ox=0, oy=0, oz=0, XYZ=None):
@@ -151,7 +152,8 @@ class D3Q7Data(C.Structure):
                 ("lbm_diffusivity", C.c_double),
                 ("df", C.c_void_p * 2), ("macro", C.c_void_p), ("map", C.c_void_p),
                 ("diffusion_coefficient", C.c_void_p), ("source", C.c_void_p),
-                ("inflow_phi", C.c_double), ("phi_ref", C.c_double), ("store_scale", C.c_double)]
+                ("inflow_phi", C.c_double), ("phi_ref", C.c_double), ("store_scale", C.c_double),
+                ("rows", C.c_void_p), ("row_count", C.c_int64 * 3)]
 
     @classmethod
     def make(cls, X, Y, Z, ox=0, oy=0, oz=0, XYZ=None):
@@ -230,6 +232,8 @@ def load_library():
     lib.tnl_lbm_last_error.restype = C.c_char_p
     lib.tnl_lbm_set_wide_addressing.argtypes = [C.c_int]
     lib.tnl_lbm_set_wide_addressing.restype = C.c_int
+    lib.tnl_lbm_classify_rows.argtypes = [C.POINTER(D3Q27Data), C.POINTER(D3Q7Data), _P, C.POINTER(C.c_int64), _P]
+    lib.tnl_lbm_classify_rows.restype = C.c_int
     lib.tnl_lbm_d3q27_reduce_macro.argtypes = [C.POINTER(Config), C.POINTER(D3Q27Data),
                                                C.POINTER(C.c_double), _P]
     lib.tnl_lbm_d3q27_reduce_macro.restype = C.c_int

@@ -441,6 +441,7 @@ class CoupledScalarRun:
         source (flat XYZ in the lattice layout); returns the buffer holding
         the final D3Q7 state."""
         try:
+            self.ade.refresh_rows()
             self.source.copy_(source.reshape(-1))
             self.ckpt[0].copy_(f0.reshape(-1))
             self.app[0].copy_(g0.reshape(-1))
@@ -468,6 +469,7 @@ class CoupledScalarRun:
         grad (flat XYZ float64) and returns dL/d(initial absolute D3Q7
         populations) (a buffer valid until the next call)."""
         try:
+            self.ade.refresh_rows()
             cur, other = self.lam
             cur.copy_(lam_T.reshape(-1))
             for j in reversed(range(self.nseg)):

@@ -309,6 +309,8 @@ def load_checkpoint(path, blocks, *, ade_blocks=(), state=None) -> dict:
                     lat.map = torch.empty(attrs["variables"][mname]["size"], dtype=torch.int16,
                                           device=lat.df[0].device)
                 load(mname, lat.map)
+                # the map changed in place: its row lists are classified again at _bind
+                lat._geo_gen = getattr(lat, "_geo_gen", 0) + 1
             else:
                 lat.map = None
             for k, t in enumerate(lat.df):

@@ -42,6 +42,17 @@ def i16_class_ranges(u_max: float, margin: float = 1.25):
                  for n in range(4))
 
 
+def classify_rows(nse: "D3Q27Lattice", ade: "D3Q7Lattice | None"):
+    """tnl_lbm_classify_rows: the row lists of nse's map (and ade's, for the
+    coupled step) as an int32 device tensor of X*Z entries and the three
+    counts (classes 0, 1, 2)."""
+    rows = torch.empty(nse.X * nse.Z, dtype=torch.int32, device=nse.device)
+    counts = (C.c_int64 * 3)()
+    nse._check(nse.lib.tnl_lbm_classify_rows(C.byref(nse.d), C.byref(ade.d) if ade is not None else None,
+                                             rows.data_ptr(), counts, _stream_ptr()))
+    return rows, tuple(counts)
+
+
 def to_storage(a: torch.Tensor) -> torch.Tensor:
     """[x, y, z] array -> contiguous storage order (x, z, y)."""
     return a.permute(0, 2, 1).contiguous()
@@ -59,7 +70,12 @@ class D3Q27Lattice:
 
     def __init__(self, X: int, Y: int, Z: int, *, collision="CUM", storage="f64", pattern="AA",
                  macro_real=None, nu: float = 0.0, force=(0.0, 0.0, 0.0), geo=None, device="cuda:0",
-                 XYZ: int | None = None, store_range: float = 1.0, store_range_class=None, overlap=(0, 0, 0)):
+                 XYZ: int | None = None, store_range: float = 1.0, store_range_class=None, overlap=(0, 0, 0),
+                 row_lists: bool = True):
+        """row_lists: classify the map's rows (tnl_lbm_classify_rows) so that
+        whole-box steps run the rows of FLUID/PERIODIC (and WALL) sites
+        through lean kernels; False steps every row through the kernel with
+        all boundary codes (same results, bit for bit)."""
         if not torch.cuda.is_available():
             raise RuntimeError("D3Q27Lattice needs a GPU (libtnl_lbm_amd.so runs on gfx950 only)")
         self.lib = _abi.load_library()
@@ -84,6 +100,8 @@ class D3Q27Lattice:
         mdt = torch.float64 if macro_real == _abi.REAL_F64 else torch.float32
         self.macro = torch.zeros(4 * n, dtype=mdt, device=self.device)
         self.map = None
+        self.row_lists = row_lists
+        self.rows, self._rows_key, self._geo_gen = None, None, 0
         self.d.lbm_viscosity = nu
         self.d.fx, self.d.fy, self.d.fz = force
         self._profiles = [None, None, None]
@@ -107,6 +125,19 @@ class D3Q27Lattice:
         self.d.macro = self.macro.data_ptr()
         self.d.map = self.map.data_ptr() if self.map is not None else None
         self.d.force_field = self.force_field.data_ptr() if getattr(self, "force_field", None) is not None else None
+        key = (self._geo_gen, self.d.map, self.row_lists)
+        if key != self._rows_key:
+            self.rows, counts = (classify_rows(self, None) if self.row_lists and self.map is not None
+                                 else (None, (0, 0, 0)))
+            self._rows_key = key
+            for c in range(3):
+                self.d.row_count[c] = counts[c]
+        self.d.rows = self.rows.data_ptr() if self.rows is not None else None
+
+    def refresh_geometry(self):
+        """After the map was changed in place: classify its rows again."""
+        self._geo_gen += 1
+        self._bind()
 
     def _check(self, rc):
         _abi.check(self.lib, rc)
@@ -119,6 +150,7 @@ class D3Q27Lattice:
             g = torch.as_tensor(geo, dtype=torch.int16)
             assert tuple(g.shape) == (self.X, self.Y, self.Z)
             self.map = to_storage(g).reshape(-1).to(self.device)
+        self._geo_gen += 1
         if hasattr(self, "iterations"):
             self._bind()
 
@@ -209,6 +241,7 @@ class D3Q27Lattice:
         update and, at the same site, the D3Q7 update driven by the NSE velocity."""
         if ade.nse is not self:
             raise ValueError("the D3Q7 lattice belongs to another NSE lattice")
+        ade.refresh_rows()
         self.update_kernel_data()
         ade.update_kernel_data(self.iterations)
         end = end or (self.X, self.Y, self.Z)
@@ -336,6 +369,7 @@ class D3Q7Lattice:
         self._mdt = torch.float64 if self.cfg.macro_real == _abi.REAL_F64 else torch.float32
         self.phi = torch.zeros(n, dtype=self._mdt, device=self.device)
         self.map = self.diffusion = self.source = None
+        self.rows, self._rows_key, self._geo_gen = None, None, 0
         if geo is not None:
             self.set_geometry(geo)
         self._bind()
@@ -351,6 +385,24 @@ class D3Q7Lattice:
         self.d.map = self.map.data_ptr() if self.map is not None else None
         self.d.diffusion_coefficient = self.diffusion.data_ptr() if self.diffusion is not None else None
         self.d.source = self.source.data_ptr() if self.source is not None else None
+        self.refresh_rows()
+
+    def refresh_rows(self):
+        """The coupled step's row lists, classified from this map and the NSE
+        lattice's, again where either has changed since."""
+        nse = self.nse
+        key = (nse._geo_gen, nse.d.map, self._geo_gen, self.d.map, nse.row_lists)
+        if key != self._rows_key:
+            self.rows, counts = classify_rows(nse, self) if nse.row_lists else (None, (0, 0, 0))
+            self._rows_key = key
+            for c in range(3):
+                self.d.row_count[c] = counts[c]
+        self.d.rows = self.rows.data_ptr() if self.rows is not None else None
+
+    def refresh_geometry(self):
+        """After the map was changed in place: classify the rows again."""
+        self._geo_gen += 1
+        self._bind()
 
     def _field(self, a):
         """[x, y, z] array -> flat device array in the lattice layout (macro_real)."""
@@ -368,6 +420,7 @@ class D3Q7Lattice:
             g = torch.as_tensor(geo, dtype=torch.int16)
             assert tuple(g.shape) == (self.X, self.Y, self.Z)
             self.map = to_storage(g).reshape(-1).to(self.device)
+        self._geo_gen += 1
         self._bind()
 
     def set_diffusivity_field(self, D):
