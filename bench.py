@@ -58,6 +58,10 @@ def parse():
                    help="also time the headline step and its streaming-only ceiling on this many further "
                         "allocations of the headline box, each made while the previous one is still held "
                         "(the achieved fraction depends on where the driver places the allocation)")
+    p.add_argument("--stride-pads", default="",
+                   help="comma-separated pads of the population-plane stride, in elements: time the "
+                        "headline step and its streaming-only ceiling at each stride on one allocation "
+                        "(made with the largest) and report them as stride_survey")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain device copy of 2 GiB (the known-size kernel that scales the "
                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -203,6 +207,35 @@ def placement_survey(N, storage, count, steps=6, tau=0.5015):
         held = lat
         del lat
     del held
+    torch.cuda.empty_cache()
+    return out
+
+
+def stride_survey(N, storage, pads, steps=8, rounds=2, tau=0.5015):
+    """The headline step and its streaming-only ceiling at several plane
+    strides XYZ = N^3 + pad on one allocation: the lattice is allocated with
+    the largest stride and its data struct takes each smaller one in turn
+    (every slot stays inside the allocation), so that the allocation's
+    placement is the same for all.  Uniform initial state."""
+    import torch
+    from pytnl_lbm import D3Q27Lattice, _abi
+
+    base = N ** 3
+    lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0, XYZ=base + max(pads))
+    lat.set_equilibrium(1.0, 0.02, -0.01, 0.015)
+    el = _abi.STORAGE_BYTES[lat.storage]
+    gb = 2 * 27 * el * base / 1e9
+    out = []
+    for r in range(rounds):
+        for pad in pads:
+            lat.d.XYZ = base + pad
+            _, c = timed_steps(lat, steps, 2, fn=lat.stream_only)
+            _, k = timed_steps(lat, steps, 2)
+            ks, cs = sum(k) / len(k), sum(c) / len(c)
+            out.append({"round": r, "pad_bytes": pad * el, "step_frac": round(gb / ks / PEAK_HBM_GBS, 4),
+                        "ceiling_frac": round(gb / cs / PEAK_HBM_GBS, 4)})
+            progress(f"stride pad {pad * el} B: step {out[-1]['step_frac']}, ceiling {out[-1]['ceiling_frac']}")
+    del lat
     torch.cuda.empty_cache()
     return out
 
@@ -570,6 +603,8 @@ def main():
     progress(f"{workload}: {head['mlups']:.0f} MLUPS, {100 * head['roofline_frac']:.1f}% of 8 TB/s")
     if args.allocations > 1:
         out["placement_survey"] = placement_survey(N, st, args.allocations - 1)
+    if args.stride_pads:
+        out["stride_survey"] = stride_survey(N, st, [int(v) for v in args.stride_pads.split(",")])
     if not args.no_shard and st == "f64" and N % args.shard_slab == 0:
         out["shard"] = shard_case(N, args.shard_slab, min(args.steps, 40), 2)
         progress(f"shard {out['shard']['workload']}: {out['shard']['mlups']:.0f} MLUPS, "
