@@ -54,10 +54,11 @@ def parse():
     p.add_argument("--storage", default="f64", choices=["f64", "f32", "f32dev", "f16dev", "bf16dev", "i16dev"],
                    help="storage form of the headline case (f64 is the headline workload; the others exist "
                         "so that each kernel can be profiled through this script)")
-    p.add_argument("--allocations", type=int, default=1,
-                   help="also time the headline step and its streaming-only ceiling on this many further "
-                        "allocations of the headline box, each made while the previous one is still held "
-                        "(the achieved fraction depends on where the driver places the allocation)")
+    p.add_argument("--allocations", type=int, default=3,
+                   help="allocations of the headline box timed in all: after the headline (the first, which "
+                        "gives `value`), the step and its streaming-only ceiling on allocations - 1 further "
+                        "ones, each made while the previous one is still held (the achieved fraction depends "
+                        "on where the driver places the allocation)")
     p.add_argument("--stride-pads", default="",
                    help="comma-separated pads of the population-plane stride, in elements: time the "
                         "headline step and its streaming-only ceiling at each stride on one allocation "

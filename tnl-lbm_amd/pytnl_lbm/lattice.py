@@ -143,12 +143,23 @@ class D3Q27Lattice:
         _abi.check(self.lib, rc)
 
     def set_geometry(self, geo):
-        """geo: int16 [x, y, z] array of GEO codes (None = all periodic)."""
+        """geo: int16 [x, y, z] array of GEO codes over the local sites (X, Y, Z),
+        or over the stored lattice with its overlaps; None = all periodic.  The
+        kernels index the map as they index the populations, so a local array
+        is placed inside the stored extents, the overlap sites coded PERIODIC
+        (the step never reads a ghost site's own code)."""
         if geo is None:
             self.map = None
         else:
             g = torch.as_tensor(geo, dtype=torch.int16)
-            assert tuple(g.shape) == (self.X, self.Y, self.Z)
+            Xs, Ys, Zs = self.X + 2 * self.ox, self.Y + 2 * self.oy, self.Z + 2 * self.oz
+            if tuple(g.shape) == (self.X, self.Y, self.Z) and (self.ox or self.oy or self.oz):
+                full = torch.full((Xs, Ys, Zs), GEO.PERIODIC, dtype=torch.int16)
+                full[self.ox:self.ox + self.X, self.oy:self.oy + self.Y, self.oz:self.oz + self.Z] = g
+                g = full
+            if tuple(g.shape) != (Xs, Ys, Zs):
+                raise ValueError(f"geometry of shape {tuple(g.shape)}: expected {(self.X, self.Y, self.Z)} or the "
+                                 f"stored {(Xs, Ys, Zs)}")
             self.map = to_storage(g).reshape(-1).to(self.device)
         self._geo_gen += 1
         if hasattr(self, "iterations"):
