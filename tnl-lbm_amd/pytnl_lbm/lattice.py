@@ -45,7 +45,14 @@ def i16_class_ranges(u_max: float, margin: float = 1.25):
 def classify_rows(nse: "D3Q27Lattice", ade: "D3Q7Lattice | None"):
     """tnl_lbm_classify_rows: the row lists of nse's map (and ade's, for the
     coupled step) as an int32 device tensor of X*Z entries and the three
-    counts (classes 0, 1, 2)."""
+    counts (classes 0, 1, 2); (None, zeros) where the lists do not apply (an
+    x or z extent of 65536 or more; a coupled pair of different geometries,
+    which the coupled step rejects itself): every row then steps through the
+    general kernel."""
+    if nse.X >= 65536 or nse.Z >= 65536:
+        return None, (0, 0, 0)
+    if ade is not None and (ade.d.ox, ade.d.oy, ade.d.oz, ade.d.XYZ) != (nse.d.ox, nse.d.oy, nse.d.oz, nse.d.XYZ):
+        return None, (0, 0, 0)
     rows = torch.empty(nse.X * nse.Z, dtype=torch.int32, device=nse.device)
     counts = (C.c_int64 * 3)()
     nse._check(nse.lib.tnl_lbm_classify_rows(C.byref(nse.d), C.byref(ade.d) if ade is not None else None,
