@@ -285,15 +285,50 @@ inline uint32_t flags_for(const DIST& distributed, bool output_macro)
 	return f;
 }
 
+// Row lists (no counterpart in the reference): a caller-owned device array
+// of local X*Z entries (a TNL array, say) and the three counts that
+// tnl_lbm_classify_rows writes.  Classify once after the map is set up and
+// again after every change of it, then pass the lists to step /
+// step_coupled: whole-box steps of the deviation forms, and every whole-box
+// coupled step, then run the rows of FLUID/PERIODIC (and WALL) sites
+// through kernels without the other boundary codes.
+struct row_lists
+{
+	uint32_t* rows = nullptr;
+	int64_t count[3] = {0, 0, 0};
+};
+
+template <typename DATA>
+inline int classify_rows(DATA& SD, row_lists& L, void* stream = nullptr)
+{
+	const tnl_lbm_d3q27_data sd = data_for(SD);
+	return tnl_lbm_classify_rows(&sd, nullptr, L.rows, L.count, stream);
+}
+
+// the coupled step's lists, from both maps
+template <typename ADE, typename NDATA, typename ADATA>
+inline int classify_rows_coupled(NDATA& nse, ADATA& ade, row_lists& L, void* stream = nullptr)
+{
+	const tnl_lbm_d3q27_data sd = data_for(nse);
+	const tnl_lbm_d3q7_data ad = ade_data_for<typename ADE::COLL>(ade);
+	return tnl_lbm_classify_rows(&sd, &ad, L.rows, L.count, stream);
+}
+
 // One launch of cudaLBMKernel<NSE>(SD, offset, end, distributed, output_macro)
 // (kernels.h:59-112): returns the library's status (0 = ok, negative =
 // error; tnl_lbm_last_error() says which).  IDX3D: anything with x(), y(), z().
+// lists: row lists of SD's map (classify_rows), or null.
 template <typename NSE, typename DATA, typename IDX3D, typename DIST>
 inline int step(DATA& SD, const IDX3D& offset, const IDX3D& end, const DIST& distributed, bool output_macro,
-				void* stream = nullptr, int storage = -1)
+				void* stream = nullptr, int storage = -1, const row_lists* lists = nullptr)
 {
 	const tnl_lbm_config cfg = config_for<NSE>(storage);
-	const tnl_lbm_d3q27_data sd = data_for(SD);
+	tnl_lbm_d3q27_data sd = data_for(SD);
+	if (lists) {
+		sd.rows = lists->rows;
+		for (int c = 0; c < 3; c++)
+			sd.row_count[c] = lists->count[c];
+	}
 	const int64_t b[3] = {(int64_t) offset.x(), (int64_t) offset.y(), (int64_t) offset.z()};
 	const int64_t e[3] = {(int64_t) end.x(), (int64_t) end.y(), (int64_t) end.z()};
 	return tnl_lbm_d3q27_step(&cfg, &sd, b, e, flags_for(distributed, output_macro), stream);
@@ -303,16 +338,22 @@ inline int step(DATA& SD, const IDX3D& offset, const IDX3D& end, const DIST& dis
 // output_macro) (kernels.h:114-196, launched at state_NSE_ADE.h:201-210):
 // the D3Q27 update and the D3Q7 update at the same site.  Both configurations
 // must share the dreal type; the storage form is the NSE one.
+// lists: row lists of both maps (classify_rows_coupled), or null.
 template <typename NSE, typename ADE, typename NDATA, typename ADATA, typename IDX3D, typename DIST>
 inline int step_coupled(NDATA& nse, ADATA& ade, const IDX3D& offset, const IDX3D& end, const DIST& distributed,
 						bool output_macro, void* stream = nullptr, int storage = -1, double phi_ref = 0.0,
-						double store_scale = 1.0)
+						double store_scale = 1.0, const row_lists* lists = nullptr)
 {
 	static_assert(std::is_same<typename NSE::TRAITS::dreal, typename ADE::TRAITS::dreal>::value,
 				  "the coupled kernel takes NSE and ADE arrays of one element type");
 	const tnl_lbm_config cfg = config_for<NSE>(storage);
 	const tnl_lbm_d3q27_data sd = data_for(nse);
-	const tnl_lbm_d3q7_data ad = ade_data_for<typename ADE::COLL>(ade, phi_ref, store_scale);
+	tnl_lbm_d3q7_data ad = ade_data_for<typename ADE::COLL>(ade, phi_ref, store_scale);
+	if (lists) {
+		ad.rows = lists->rows;
+		for (int c = 0; c < 3; c++)
+			ad.row_count[c] = lists->count[c];
+	}
 	const int64_t b[3] = {(int64_t) offset.x(), (int64_t) offset.y(), (int64_t) offset.z()};
 	const int64_t e[3] = {(int64_t) end.x(), (int64_t) end.y(), (int64_t) end.z()};
 	return tnl_lbm_nse_ade_step(&cfg, &sd, &ad, b, e, flags_for(distributed, output_macro), stream);

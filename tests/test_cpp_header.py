@@ -98,6 +98,21 @@ template <typename N, typename DATA>
 int unused_launch(DATA& d) { return tnl_lbm_amd::step<N>(d, Idx3{0, 0, 0}, Idx3{5, 6, 7}, Dist{false, false, false}, true); }
 template <typename N, typename A, typename D1, typename D2>
 int unused_coupled(D1& n, D2& a) { return tnl_lbm_amd::step_coupled<N, A>(n, a, Idx3{0, 0, 0}, Idx3{5, 6, 7}, Dist{false, false, false}, true); }
+template <typename N, typename DATA>
+int unused_lists(DATA& d, tnl_lbm_amd::row_lists& L)
+{
+    if (int rc = tnl_lbm_amd::classify_rows(d, L))
+        return rc;
+    return tnl_lbm_amd::step<N>(d, Idx3{0, 0, 0}, Idx3{5, 6, 7}, Dist{false, false, false}, true, nullptr, -1, &L);
+}
+template <typename N, typename A, typename D1, typename D2>
+int unused_coupled_lists(D1& n, D2& a, tnl_lbm_amd::row_lists& L)
+{
+    if (int rc = tnl_lbm_amd::classify_rows_coupled<A>(n, a, L))
+        return rc;
+    return tnl_lbm_amd::step_coupled<N, A>(n, a, Idx3{0, 0, 0}, Idx3{5, 6, 7}, Dist{false, false, false}, true,
+                                           nullptr, -1, 0.0, 1.0, &L);
+}
 
 int main()
 {
@@ -135,6 +150,9 @@ int main()
     if (false) {
         unused_launch<NSE<TraitsDP, D3Q27_CUM<TraitsDP>>>(a);
         unused_coupled<NSE<TraitsSP, D3Q27_CUM<TraitsSP>>, NSE<TraitsSP, D3Q7_SRT<TraitsSP>>>(c, e);
+        row_lists L;
+        unused_lists<NSE<TraitsDP, D3Q27_CUM<TraitsDP>>>(a, L);
+        unused_coupled_lists<NSE<TraitsSP, D3Q27_CUM<TraitsSP>>, NSE<TraitsSP, D3Q7_SRT<TraitsSP>>>(c, e, L);
     }
     return 0;
 }
