@@ -94,6 +94,66 @@ struct Ade7
 		return phi;
 	}
 
+	// mu = C^T lam for the collision f -> C f of a given velocity (phi = sum f
+	// is a linear functional of f).  SRT is (1 - omega) I plus the rank-one
+	// omega (w E) 1^T; MRT and CLBM are I plus, per axis, two rank-one terms
+	// (relaxed moment functional) x (update vector) read off collide below,
+	// so the transpose costs about one collision instead of seven.
+	static LBM_DEV void collide_adjoint(int coll, const real* lam, real* mu, real vx, real vy, real vz, real D)
+	{
+		const real omega = (real) 1 / ((real) 0.5 + (real) 4 * D);
+		if (coll == TNL_LBM_ADE_COLL_SRT) {
+			// f' = (1 - omega) f + omega w_q E_q(u) phi (col_srt.h:13-26)
+			const real S = eq<0>((real) 1, vx, vy, vz) * lam[0] + eq<1>((real) 1, vx, vy, vz) * lam[1]
+						 + eq<2>((real) 1, vx, vy, vz) * lam[2] + eq<3>((real) 1, vx, vy, vz) * lam[3]
+						 + eq<4>((real) 1, vx, vy, vz) * lam[4] + eq<5>((real) 1, vx, vy, vz) * lam[5]
+						 + eq<6>((real) 1, vx, vy, vz) * lam[6];
+#pragma unroll
+			for (int q = 0; q < 7; q++)
+				mu[q] = ((real) 1 - omega) * lam[q] + omega * S;
+			return;
+		}
+		// per axis a (velocity v, + and - directions p and m):
+		//   first moment  alpha = v 1 + e_m - e_p (relaxed with omega),
+		//   second moment beta  = MRT: (v^2 + Cs2) 1 - e_m - e_p
+		//                         CLBM: (Cs2 - v^2) 1 + (2v - 1) e_p - (2v + 1) e_m,
+		// update vectors b (first) and c = -e_0 + (e_p + e_m)/2 (second), with
+		//   b = MRT: (e_p - e_m)/2,  CLBM: -2v e_0 + (v + 1/2) e_p + (v - 1/2) e_m
+		// (col_mrt.h:15-50, col_clbm.h:15-84); C^T lam = lam + sum_a
+		// [omega (b.lam) alpha + (c.lam) beta].
+		const bool clbm = coll != TNL_LBM_ADE_COLL_MRT;
+		const real Cs2 = (real) 0.25, h = (real) 0.5;
+		const real v[3] = {vx, vy, vz};
+		real s1[3], s2[3];
+		real common = 0;
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+			const int p = 1 + 2 * a, m = 2 + 2 * a;
+			s2[a] = h * (lam[p] + lam[m]) - lam[0];
+			if (clbm) {
+				s1[a] = omega * ((v[a] + h) * lam[p] + (v[a] - h) * lam[m] - (real) 2 * v[a] * lam[0]);
+				common += s1[a] * v[a] + s2[a] * (Cs2 - v[a] * v[a]);
+			}
+			else {
+				s1[a] = omega * h * (lam[p] - lam[m]);
+				common += s1[a] * v[a] + s2[a] * (v[a] * v[a] + Cs2);
+			}
+		}
+		mu[0] = lam[0] + common;
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+			const int p = 1 + 2 * a, m = 2 + 2 * a;
+			if (clbm) {
+				mu[p] = lam[p] + common - s1[a] + s2[a] * ((real) 2 * v[a] - (real) 1);
+				mu[m] = lam[m] + common + s1[a] - s2[a] * ((real) 2 * v[a] + (real) 1);
+			}
+			else {
+				mu[p] = lam[p] + common - s1[a] - s2[a];
+				mu[m] = lam[m] + common + s1[a] - s2[a];
+			}
+		}
+	}
+
 	static LBM_DEV void collide(int coll, real* f, real phi, real vx, real vy, real vz, real D)
 	{
 		const real omega = (real) 1 / ((real) 0.5 + (real) 4 * D);

@@ -465,7 +465,8 @@ __global__ void __launch_bounds__(256, 2) k_adjoint_ab(const StepArgs A, const A
 // relax moments whose equilibria are linear in phi), so
 //   lambda <- C(u)^T lambda_next scattered back along the pull,
 //   dJ/dS  += sum_q w_q lambda_next_q at the sites that collide.
-// The columns of C(u) are the forward collision applied to the unit vectors.
+// C(u)^T is applied directly (Ade7::collide_adjoint: each operator is the
+// identity or a multiple of it plus a few rank-one terms).
 // ===========================================================================
 struct AdeAdjArgs
 {
@@ -545,19 +546,7 @@ __global__ void __launch_bounds__(256) k_ade_adjoint_ab(const StepArgs A, const 
 					};
 					const real vx = macro_at(P.u, s0), vy = macro_at(P.u, A.XYZ + s0), vz = macro_at(P.u, 2 * A.XYZ + s0);
 					const real D = P.diff ? macro_at(P.diff, s0) : (real) P.D;
-#pragma unroll
-					for (int p = 0; p < 7; p++) {
-						real e[7];
-#pragma unroll
-						for (int q = 0; q < 7; q++)
-							e[q] = q == p ? (real) 1 : (real) 0;
-						K::collide(P.collision, e, (real) 1, vx, vy, vz, D);
-						real acc = 0;
-#pragma unroll
-						for (int q = 0; q < 7; q++)
-							acc += lam[q] * e[q];
-						mu[p] = acc;
-					}
+					K::collide_adjoint(P.collision, lam, mu, vx, vy, vz, D);
 					if (! SPECIAL && P.grad_source) {
 						real g = 0;
 #pragma unroll
