@@ -54,6 +54,10 @@ def parse():
     p.add_argument("--storage", default="f64", choices=["f64", "f32", "f32dev", "f16dev", "bf16dev", "i16dev"],
                    help="storage form of the headline case (f64 is the headline workload; the others exist "
                         "so that each kernel can be profiled through this script)")
+    p.add_argument("--placement", default="auto", choices=("auto", "first"),
+                   help="population buffer placement of the benchmark lattices: the library's default (auto: "
+                        "buffers of 4 GiB or more are probed and a slow placement is replaced) or the first "
+                        "allocation as it lands")
     p.add_argument("--allocations", type=int, default=3,
                    help="allocations of the headline box timed in all: after the headline (the first, which "
                         "gives `value`), the step and its streaming-only ceiling on allocations - 1 further "
@@ -95,12 +99,13 @@ def timed_steps(lat, steps, warmup, fn=None):
     return wall, kern
 
 
-def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015, store_range=1.0, ceiling=True):
+def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015, store_range=1.0, ceiling=True,
+               placement="auto"):
     import torch
     from pytnl_lbm import D3Q27Lattice, _abi
     from pytnl_lbm.fields import turbulence_fields_torch
 
-    lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0, store_range=store_range)
+    lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0, store_range=store_range, placement=placement)
     u = turbulence_fields_torch(N, seed, k0, u_rms, device="cuda")
     rho = torch.ones_like(u[0])
     lat.set_equilibrium_field(rho, *u)
@@ -120,6 +125,10 @@ def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015
         "achieved_GBs": bytes_per_site * sites / avg_kernel / 1e9,
     }
     res["roofline_frac"] = res["achieved_GBs"] / PEAK_HBM_GBS
+    if lat.placement is not None:
+        res["placement"] = {"policy": placement, "probed_frac": [round(r / PEAK_HBM_GBS, 4)
+                                                                 for r in lat.placement["probed_GBs"]],
+                            "kept": lat.placement["kept"]}
     # FP64 reductions of the final state (mass, NaN check)
     lat.compute_macro()
     red = lat.reduce_macro()
@@ -186,23 +195,28 @@ def shard_case(N, slab, steps, warmup, tau=0.5015):
 
 def placement_survey(N, storage, count, steps=6, tau=0.5015):
     """The headline step and the streaming-only step on `count` successive
-    allocations of the N^3 box, each made while the previous one is held (so
-    the driver places it elsewhere); uniform initial state.  Per allocation:
-    fraction of 8 TB/s for both, and their ratio."""
+    allocations of the N^3 box, each kept as allocated (no placement probing)
+    and made while the previous one is held (so the driver places it
+    elsewhere); uniform initial state.  Per allocation: the placement probe's
+    rate right after the allocation, then the fraction of 8 TB/s of both
+    steps, and their ratio."""
     import torch
     from pytnl_lbm import D3Q27Lattice, _abi
 
     out, held = [], None
     for i in range(count):
-        lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0)
+        lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0, placement="first")
+        size = lat.df[0].numel() * lat.df[0].element_size()
+        probe = lat._probe_rate(lat.df[0], size) / PEAK_HBM_GBS
         lat.set_equilibrium(1.0, 0.02, -0.01, 0.015)
         _, k = timed_steps(lat, steps, 2)
         _, c = timed_steps(lat, steps, 2, fn=lat.stream_only)
         gb = 2 * 27 * _abi.STORAGE_BYTES[lat.storage] * N ** 3 / 1e9
         ks, cs = sum(k) / len(k), sum(c) / len(c)
-        out.append({"allocation": i + 1, "step_frac": round(gb / ks / PEAK_HBM_GBS, 4),
+        out.append({"allocation": i + 1, "probe_frac": round(probe, 4), "step_frac": round(gb / ks / PEAK_HBM_GBS, 4),
                     "ceiling_frac": round(gb / cs / PEAK_HBM_GBS, 4), "step_rate_over_ceiling_rate": round(cs / ks, 4)})
-        progress(f"allocation {i + 1}: step {out[-1]['step_frac']}, ceiling {out[-1]['ceiling_frac']}")
+        progress(f"allocation {i + 1}: probe {out[-1]['probe_frac']}, step {out[-1]['step_frac']}, "
+                 f"ceiling {out[-1]['ceiling_frac']}")
         del held
         torch.cuda.empty_cache()
         held = lat
@@ -580,7 +594,8 @@ def main():
     N = args.size
     st = args.storage
     tau = 0.5015 if st == "f64" else 0.505
-    head = bench_case(N, st, args.steps, args.warmup, tau=tau, store_range=0.5 if st == "i16dev" else 1.0)
+    head = bench_case(N, st, args.steps, args.warmup, tau=tau, store_range=0.5 if st == "i16dev" else 1.0,
+                      placement=args.placement)
     kernel = (f"k_aa_even_pair<{st.upper()},CUM> (even) / k_aa_periodic<{st.upper()},CUM> (odd)"
               if st in ("f16dev", "bf16dev", "i16dev") else f"k_aa_periodic<{st.upper()},CUM> (even/odd)")
     workload = f"D3Q27 cumulant A-A {st.upper()}, {N}^3 periodic box, tau={tau}"
@@ -607,6 +622,7 @@ def main():
                      "kernel_ms_odd": round(head["kernel_ms_odd"], 3)},
         "checks": {"mass_rel_drift": head["mass_rel_drift"], "nan_sites": head["nan_sites"]},
         "ceiling": head.get("ceiling"),
+        "placement": head.get("placement"),
     }
     progress(f"{workload}: {head['mlups']:.0f} MLUPS, {100 * head['roofline_frac']:.1f}% of 8 TB/s")
     if args.allocations > 1:
@@ -621,13 +637,14 @@ def main():
         per = {}
         P = args.prec_size
         for pst in ("f32", "f32dev", "f16dev", "bf16dev", "i16dev"):
-            r = bench_case(P, pst, args.prec_steps, args.warmup, tau=0.505, store_range=0.5 if pst == "i16dev" else 1.0)
+            r = bench_case(P, pst, args.prec_steps, args.warmup, tau=0.505, store_range=0.5 if pst == "i16dev" else 1.0,
+                           placement=args.placement)
             per[f"{pst}_{P}"] = {"mlups": round(r["mlups"], 1), "achieved_GBs": round(r["achieved_GBs"], 1),
                                  "roofline_frac": round(r["roofline_frac"], 4), "bytes_per_site": r["bytes_per_site"],
                                  "kernel_ms_avg": round(r["kernel_ms_avg"], 3),
                                  "kernel_ms_even": round(r["kernel_ms_even"], 3),
                                  "kernel_ms_odd": round(r["kernel_ms_odd"], 3),
-                                 "ceiling": r.get("ceiling")}
+                                 "ceiling": r.get("ceiling"), "placement": r.get("placement")}
             progress(f"{pst} {P}^3: {r['mlups']:.0f} MLUPS, {100 * r['roofline_frac']:.1f}% of 8 TB/s")
         spec = []
         for ic in ("tgv", "turbulence"):

@@ -22,6 +22,18 @@ _TORCH_ELEM = {_abi.STORE_F64: torch.float64, _abi.STORE_F32: torch.float32, _ab
                _abi.STORE_F16_DEV: torch.float16, _abi.STORE_BF16_DEV: torch.bfloat16, _abi.STORE_I16_DEV: torch.int16}
 
 
+# Placement probing of the population buffer (D3Q27Lattice._allocate_populations):
+# buffers of at least this size are probed under placement="auto"
+PLACEMENT_MIN_BYTES = 4 << 30
+# a candidate whose streaming-only step reaches this rate (0.75 of 8 TB/s) is
+# kept without trying another: the A-A streaming kernels measured 0.63-0.80 of
+# 8 TB/s over 768^3 FP64 buffers, from one allocation to the next and from one
+# device to another
+PLACEMENT_GOOD_GBS = 6000.0
+# device memory left free after a further candidate
+PLACEMENT_RESERVE_BYTES = 16 << 30
+
+
 def _stream_ptr(stream=None):
     s = stream if stream is not None else torch.cuda.current_stream()
     return C.c_void_p(s.cuda_stream)
@@ -78,11 +90,17 @@ class D3Q27Lattice:
     def __init__(self, X: int, Y: int, Z: int, *, collision="CUM", storage="f64", pattern="AA",
                  macro_real=None, nu: float = 0.0, force=(0.0, 0.0, 0.0), geo=None, device="cuda:0",
                  XYZ: int | None = None, store_range: float = 1.0, store_range_class=None, overlap=(0, 0, 0),
-                 row_lists: bool = True):
+                 row_lists: bool = True, placement: str = "auto", placement_tries: int = 4):
         """row_lists: classify the map's rows (tnl_lbm_classify_rows) so that
         whole-box steps run the rows of FLUID/PERIODIC (and WALL) sites
         through lean kernels; False steps every row through the kernel with
-        all boundary codes (same results, bit for bit)."""
+        all boundary codes (same results, bit for bit).
+
+        placement: "first" keeps the first population buffer allocated;
+        "probe" times candidate buffers and keeps the fastest (up to
+        placement_tries of them, see _allocate_populations); "auto" probes
+        A-A lattices without overlaps whose population buffer holds at least
+        PLACEMENT_MIN_BYTES, and keeps the first buffer otherwise."""
         if not torch.cuda.is_available():
             raise RuntimeError("D3Q27Lattice needs a GPU (libtnl_lbm_amd.so runs on gfx950 only)")
         self.lib = _abi.load_library()
@@ -103,7 +121,8 @@ class D3Q27Lattice:
         n = self.d.XYZ
         el = _TORCH_ELEM[st]
         ncop = 2 if pat == _abi.PATTERN_AB else 1
-        self.df = [torch.empty(27 * n, dtype=el, device=self.device) for _ in range(ncop)]
+        self.d.lbm_viscosity = nu if nu else 1.0 / 6.0  # the probe's launches need a non-zero value
+        self.df = self._allocate_populations(ncop, el, n, placement, placement_tries)
         mdt = torch.float64 if macro_real == _abi.REAL_F64 else torch.float32
         self.macro = torch.zeros(4 * n, dtype=mdt, device=self.device)
         self.map = None
@@ -116,6 +135,68 @@ class D3Q27Lattice:
             self.set_geometry(geo)
         self.iterations = 0
         self._bind()
+
+    # ---- population buffers ----------------------------------------------------
+    def _allocate_populations(self, ncop, el, n, placement, tries):
+        """The population buffers.  Where a buffer lands in HBM changes from
+        one allocation to the next and stays for the allocation's lifetime:
+        the same A-A step over two buffers of one size can run at different
+        rates (768^3 FP64: 0.76-0.79 of 8 TB/s on most allocations, 0.63-0.66
+        on some).  A probed candidate is timed right after its allocation by
+        two untimed and four timed streaming-only steps (its contents are
+        undefined until the populations are set).  Until the fastest so far
+        reaches PLACEMENT_GOOD_GBS, a further candidate is allocated while
+        that one is held (so that it lands elsewhere) and the slower of the
+        two is freed and handed back to the driver (torch.cuda.empty_cache,
+        so that the next candidate is not served from torch's cache), up to
+        `tries` candidates and as long as the device has room for one more,
+        the macro array and PLACEMENT_RESERVE_BYTES.  self.placement records
+        the probed rates in GB/s (algorithmic bytes of a step over its
+        duration) and the index kept."""
+        size = 27 * n * torch.tensor([], dtype=el).element_size()
+        if placement not in ("auto", "probe", "first"):
+            raise ValueError(f"placement {placement!r}: expected 'auto', 'probe' or 'first'")
+        probe_ok = self.cfg.pattern == _abi.PATTERN_AA and (self.ox, self.oy, self.oz) == (0, 0, 0)
+        if placement == "probe" and not probe_ok:
+            raise ValueError("placement probing runs the periodic A-A streaming kernels: an A-A lattice "
+                             "without overlaps only")
+        probe = probe_ok and (placement == "probe" or (placement == "auto" and size >= PLACEMENT_MIN_BYTES))
+        self.placement = None
+        if not probe:
+            return [torch.empty(27 * n, dtype=el, device=self.device) for _ in range(ncop)]
+        best = torch.empty(27 * n, dtype=el, device=self.device)
+        rates = [self._probe_rate(best, size)]
+        keep = 0
+        while rates[keep] < PLACEMENT_GOOD_GBS and len(rates) < tries:
+            if torch.cuda.mem_get_info(self.device)[0] < size + 4 * n * 8 + PLACEMENT_RESERVE_BYTES:
+                break
+            cand = torch.empty(27 * n, dtype=el, device=self.device)
+            rates.append(self._probe_rate(cand, size))
+            if rates[-1] > rates[keep]:
+                best, keep = cand, len(rates) - 1
+            del cand
+            # the slower of the two goes back to the driver, not to torch's cache,
+            # from which the next candidate of this size would be served
+            torch.cuda.empty_cache()
+        self.placement = {"probed_GBs": [round(r, 1) for r in rates], "kept": keep}
+        return [best]
+
+    def _probe_rate(self, buf, size):
+        """Algorithmic GB/s of streaming-only A-A steps over `buf` (even and
+        odd alternating), timed with events on the current stream."""
+        saved = self.d.df[0], self.d.df[1], self.d.even_iter
+        self.d.df[0], self.d.df[1] = buf.data_ptr(), None
+        s = torch.cuda.current_stream(self.device)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        for i in range(6):
+            if i == 2:
+                ev[0].record(s)
+            self.d.even_iter = 1 - i % 2
+            self._check(self.lib.tnl_lbm_d3q27_stream_only(C.byref(self.cfg), C.byref(self.d), _stream_ptr(s)))
+        ev[1].record(s)
+        ev[1].synchronize()
+        self.d.df[0], self.d.df[1], self.d.even_iter = saved
+        return 2 * size * 4 / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e9
 
     # ---- bookkeeping --------------------------------------------------------
     @property
