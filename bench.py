@@ -35,7 +35,7 @@ METRIC = "D3Q27 A-A MLUPS & %HBM roofline per precision; adjoint fwd+bwd MLUPS"
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=200)
+    p.add_argument("--steps", type=int, default=500, help="timed steps of the FP64 768^3 headline (the workload is quoted on 500)")
     p.add_argument("--warmup", type=int, default=10)
     p.add_argument("--size", type=int, default=768, help="edge of the FP64 headline box")
     p.add_argument("--prec-size", type=int, default=512, help="edge of the per-precision boxes")
