@@ -32,6 +32,12 @@ PLACEMENT_MIN_BYTES = 4 << 30
 PLACEMENT_GOOD_GBS = 6000.0
 # device memory left free after a further candidate
 PLACEMENT_RESERVE_BYTES = 16 << 30
+# spacer allocations made before the second, third, ... candidate: where a
+# 98 GB buffer lands depends on what is allocated before it, and candidates
+# allocated in the same state repeat the same few placements (768^3 FP64, one
+# box: 63.8, 66.1, 70.0 and 63.9% in turn without spacers; after a spacer
+# 63.8-78.1%, changing with the spacer's size and with what is held)
+PLACEMENT_SPACERS = (6 << 30, 30 << 30, 14 << 30, 0, 2 << 30)
 
 
 def _stream_ptr(stream=None):
@@ -90,7 +96,7 @@ class D3Q27Lattice:
     def __init__(self, X: int, Y: int, Z: int, *, collision="CUM", storage="f64", pattern="AA",
                  macro_real=None, nu: float = 0.0, force=(0.0, 0.0, 0.0), geo=None, device="cuda:0",
                  XYZ: int | None = None, store_range: float = 1.0, store_range_class=None, overlap=(0, 0, 0),
-                 row_lists: bool = True, placement: str = "auto", placement_tries: int = 4):
+                 row_lists: bool = True, placement: str = "auto", placement_tries: int = 6):
         """row_lists: classify the map's rows (tnl_lbm_classify_rows) so that
         whole-box steps run the rows of FLUID/PERIODIC (and WALL) sites
         through lean kernels; False steps every row through the kernel with
@@ -146,11 +152,13 @@ class D3Q27Lattice:
         two untimed and four timed streaming-only steps (its contents are
         undefined until the populations are set).  Until the fastest so far
         reaches PLACEMENT_GOOD_GBS, a further candidate is allocated while
-        that one is held (so that it lands elsewhere) and the slower of the
-        two is freed and handed back to the driver (torch.cuda.empty_cache,
-        so that the next candidate is not served from torch's cache), up to
-        `tries` candidates and as long as the device has room for one more,
-        the macro array and PLACEMENT_RESERVE_BYTES.  self.placement records
+        that one is held, right after a spacer allocation of the next size in
+        PLACEMENT_SPACERS (so that it starts elsewhere; the spacer is freed
+        once the candidate exists), and the slower of the two is freed and
+        handed back to the driver (torch.cuda.empty_cache, so that the next
+        allocation is not served from torch's cache), up to `tries`
+        candidates and as long as the device has room for the spacer, one
+        more candidate, the macro array and PLACEMENT_RESERVE_BYTES.  self.placement records
         the probed rates in GB/s (algorithmic bytes of a step over its
         duration) and the index kept."""
         size = 27 * n * torch.tensor([], dtype=el).element_size()
@@ -168,15 +176,18 @@ class D3Q27Lattice:
         rates = [self._probe_rate(best, size)]
         keep = 0
         while rates[keep] < PLACEMENT_GOOD_GBS and len(rates) < tries:
-            if torch.cuda.mem_get_info(self.device)[0] < size + 4 * n * 8 + PLACEMENT_RESERVE_BYTES:
+            spacer = PLACEMENT_SPACERS[(len(rates) - 1) % len(PLACEMENT_SPACERS)]
+            if torch.cuda.mem_get_info(self.device)[0] < spacer + size + 4 * n * 8 + PLACEMENT_RESERVE_BYTES:
                 break
+            sp = torch.empty(spacer, dtype=torch.uint8, device=self.device) if spacer else None
             cand = torch.empty(27 * n, dtype=el, device=self.device)
+            del sp
             rates.append(self._probe_rate(cand, size))
             if rates[-1] > rates[keep]:
                 best, keep = cand, len(rates) - 1
             del cand
-            # the slower of the two goes back to the driver, not to torch's cache,
-            # from which the next candidate of this size would be served
+            # the spacer and the slower candidate go back to the driver, not to
+            # torch's cache, from which the next allocation would be served
             torch.cuda.empty_cache()
         self.placement = {"probed_GBs": [round(r, 1) for r in rates], "kept": keep}
         return [best]
