@@ -40,6 +40,9 @@ def parse():
     p.add_argument("--size", type=int, default=768, help="edge of the FP64 headline box")
     p.add_argument("--prec-size", type=int, default=512, help="edge of the per-precision boxes")
     p.add_argument("--prec-steps", type=int, default=100)
+    p.add_argument("--prec-warmup", type=int, default=30,
+                   help="untimed steps before each per-precision timing (the 2-byte odd steps take their first "
+                        "10-15 launches to settle)")
     p.add_argument("--err-size", type=int, default=512, help="edge of the boxes of the error report")
     p.add_argument("--err-steps", type=int, default=2000)
     p.add_argument("--no-precision", action="store_true")
@@ -637,8 +640,8 @@ def main():
         per = {}
         P = args.prec_size
         for pst in ("f32", "f32dev", "f16dev", "bf16dev", "i16dev"):
-            r = bench_case(P, pst, args.prec_steps, args.warmup, tau=0.505, store_range=0.5 if pst == "i16dev" else 1.0,
-                           placement=args.placement)
+            r = bench_case(P, pst, args.prec_steps, args.prec_warmup, tau=0.505,
+                           store_range=0.5 if pst == "i16dev" else 1.0, placement=args.placement)
             per[f"{pst}_{P}"] = {"mlups": round(r["mlups"], 1), "achieved_GBs": round(r["achieved_GBs"], 1),
                                  "roofline_frac": round(r["roofline_frac"], 4), "bytes_per_site": r["bytes_per_site"],
                                  "kernel_ms_avg": round(r["kernel_ms_avg"], 3),
