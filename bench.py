@@ -155,7 +155,7 @@ def bench_case(N, storage, steps, warmup, seed=1, u_rms=0.04, k0=8.0, tau=0.5015
     return res
 
 
-def shard_case(N, slab, steps, warmup, tau=0.5015):
+def shard_case(N, slab, steps, warmup, tau=0.5015, placement="auto"):
     """One GPU's share of the FP64 N^3 box split into x-slabs over N/slab
     GPUs: slab x N x N sites plus a ghost plane per side, stepped with
     DISTRIBUTED_X.  The slab is its own periodic neighbour, so the halo
@@ -166,7 +166,7 @@ def shard_case(N, slab, steps, warmup, tau=0.5015):
     import torch
     from pytnl_lbm import D3Q27Lattice, _abi
 
-    lat = D3Q27Lattice(slab, N, N, storage="f64", nu=(tau - 0.5) / 3.0, overlap=(1, 0, 0))
+    lat = D3Q27Lattice(slab, N, N, storage="f64", nu=(tau - 0.5) / 3.0, overlap=(1, 0, 0), placement=placement)
     lat.set_equilibrium(1.0, 0.02, -0.01, 0.015)
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
     for i in range(warmup + steps):
@@ -191,6 +191,9 @@ def shard_case(N, slab, steps, warmup, tau=0.5015):
            "halo_ms_avg": round(h * 1e3, 4), "halo_bytes_per_step": 2 * face,
            "halo_note": "device-local copies of the 2 x 9 boundary planes; over one xGMI link per face "
                         f"(~153 GB/s) a face would take ~{face / 153e9 * 1e3:.2f} ms, overlappable with the interior"}
+    if lat.placement is not None:
+        res["placement"] = {"probed_frac": [round(r / PEAK_HBM_GBS, 4) for r in lat.placement["probed_GBs"]],
+                            "kept": lat.placement["kept"]}
     del lat
     torch.cuda.empty_cache()
     return res
@@ -209,8 +212,7 @@ def placement_survey(N, storage, count, steps=6, tau=0.5015):
     out, held = [], None
     for i in range(count):
         lat = D3Q27Lattice(N, N, N, storage=storage, nu=(tau - 0.5) / 3.0, placement="first")
-        size = lat.df[0].numel() * lat.df[0].element_size()
-        probe = lat._probe_rate(lat.df[0], size) / PEAK_HBM_GBS
+        probe = lat._probe_rate(lat.df[0]) / PEAK_HBM_GBS
         lat.set_equilibrium(1.0, 0.02, -0.01, 0.015)
         _, k = timed_steps(lat, steps, 2)
         _, c = timed_steps(lat, steps, 2, fn=lat.stream_only)
@@ -633,7 +635,7 @@ def main():
     if args.stride_pads:
         out["stride_survey"] = stride_survey(N, st, [int(v) for v in args.stride_pads.split(",")])
     if not args.no_shard and st == "f64" and N % args.shard_slab == 0:
-        out["shard"] = shard_case(N, args.shard_slab, min(args.steps, 40), 2)
+        out["shard"] = shard_case(N, args.shard_slab, min(args.steps, 40), 2, placement=args.placement)
         progress(f"shard {out['shard']['workload']}: {out['shard']['mlups']:.0f} MLUPS, "
                  f"halo {out['shard']['halo_ms_avg']} ms per step")
     if not args.no_precision:

@@ -54,7 +54,20 @@ def test_probe_arguments_are_checked():
         D3Q27Lattice(16, 16, 16, nu=0.01, placement="fastest")
     with pytest.raises(ValueError):
         D3Q27Lattice(16, 16, 16, nu=0.01, pattern="AB", placement="probe")
-    with pytest.raises(ValueError):
-        D3Q27Lattice(16, 16, 16, nu=0.01, overlap=(1, 0, 0), placement="probe")
-    # auto on an A-B lattice or one with overlaps keeps the first allocation
+    # auto on an A-B lattice keeps the first allocation
     assert D3Q27Lattice(16, 16, 16, nu=0.01, pattern="AB").placement is None
+
+
+def test_probed_slab_with_overlaps_steps_like_the_first_allocation():
+    """A slab with ghost planes (the x-distributed share) is probed over its
+    local sites; the distributed step afterwards matches the unprobed one."""
+    out = []
+    for placement in ("first", "probe"):
+        lat = D3Q27Lattice(12, 20, 24, nu=0.01, overlap=(1, 0, 0), placement=placement, placement_tries=2)
+        assert (lat.placement is None) == (placement == "first")
+        lat.set_equilibrium(1.0, 0.02, 0.01, -0.03)
+        for _ in range(3):
+            lat.step()
+        torch.cuda.synchronize()
+        out.append(lat.export_f64().cpu())
+    assert torch.equal(out[0], out[1])

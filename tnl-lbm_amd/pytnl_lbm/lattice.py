@@ -105,7 +105,7 @@ class D3Q27Lattice:
         placement: "first" keeps the first population buffer allocated;
         "probe" times candidate buffers and keeps the fastest (up to
         placement_tries of them, see _allocate_populations); "auto" probes
-        A-A lattices without overlaps whose population buffer holds at least
+        A-A lattices whose population buffer holds at least
         PLACEMENT_MIN_BYTES, and keeps the first buffer otherwise."""
         if not torch.cuda.is_available():
             raise RuntimeError("D3Q27Lattice needs a GPU (libtnl_lbm_amd.so runs on gfx950 only)")
@@ -164,16 +164,15 @@ class D3Q27Lattice:
         size = 27 * n * torch.tensor([], dtype=el).element_size()
         if placement not in ("auto", "probe", "first"):
             raise ValueError(f"placement {placement!r}: expected 'auto', 'probe' or 'first'")
-        probe_ok = self.cfg.pattern == _abi.PATTERN_AA and (self.ox, self.oy, self.oz) == (0, 0, 0)
+        probe_ok = self.cfg.pattern == _abi.PATTERN_AA
         if placement == "probe" and not probe_ok:
-            raise ValueError("placement probing runs the periodic A-A streaming kernels: an A-A lattice "
-                             "without overlaps only")
+            raise ValueError("placement probing runs the periodic A-A streaming kernels: an A-A lattice only")
         probe = probe_ok and (placement == "probe" or (placement == "auto" and size >= PLACEMENT_MIN_BYTES))
         self.placement = None
         if not probe:
             return [torch.empty(27 * n, dtype=el, device=self.device) for _ in range(ncop)]
         best = torch.empty(27 * n, dtype=el, device=self.device)
-        rates = [self._probe_rate(best, size)]
+        rates = [self._probe_rate(best)]
         keep = 0
         while rates[keep] < PLACEMENT_GOOD_GBS and len(rates) < tries:
             spacer = PLACEMENT_SPACERS[(len(rates) - 1) % len(PLACEMENT_SPACERS)]
@@ -182,7 +181,7 @@ class D3Q27Lattice:
             sp = torch.empty(spacer, dtype=torch.uint8, device=self.device) if spacer else None
             cand = torch.empty(27 * n, dtype=el, device=self.device)
             del sp
-            rates.append(self._probe_rate(cand, size))
+            rates.append(self._probe_rate(cand))
             if rates[-1] > rates[keep]:
                 best, keep = cand, len(rates) - 1
             del cand
@@ -192,9 +191,10 @@ class D3Q27Lattice:
         self.placement = {"probed_GBs": [round(r, 1) for r in rates], "kept": keep}
         return [best]
 
-    def _probe_rate(self, buf, size):
+    def _probe_rate(self, buf):
         """Algorithmic GB/s of streaming-only A-A steps over `buf` (even and
-        odd alternating), timed with events on the current stream."""
+        odd alternating, the local sites without the overlaps), timed with
+        events on the current stream."""
         saved = self.d.df[0], self.d.df[1], self.d.even_iter
         self.d.df[0], self.d.df[1] = buf.data_ptr(), None
         s = torch.cuda.current_stream(self.device)
@@ -207,7 +207,8 @@ class D3Q27Lattice:
         ev[1].record(s)
         ev[1].synchronize()
         self.d.df[0], self.d.df[1], self.d.even_iter = saved
-        return 2 * size * 4 / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e9
+        step_bytes = 2 * 27 * buf.element_size() * self.X * self.Y * self.Z
+        return step_bytes * 4 / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e9
 
     # ---- bookkeeping --------------------------------------------------------
     @property
