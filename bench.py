@@ -13,6 +13,10 @@ pattern, default cumulant.  Beside it:
   against forward throughput, with central-difference checks;
 - the CPU oracle (a restatement of the reference's FP64 A-B host path) timed
   on a bounded sample on the host cores.
+The benchmark lattices take the library's default placement probing of
+their population buffers (`placement` in the JSON line; --placement first
+keeps the first allocation), and `placement_survey` times further
+allocations as they land.
 
 Prints ONE JSON line.
 """
