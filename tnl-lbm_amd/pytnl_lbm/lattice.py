@@ -147,20 +147,21 @@ class D3Q27Lattice:
         """The population buffers.  Where a buffer lands in HBM changes from
         one allocation to the next and stays for the allocation's lifetime:
         the same A-A step over two buffers of one size can run at different
-        rates (768^3 FP64: 0.76-0.79 of 8 TB/s on most allocations, 0.63-0.66
-        on some).  A probed candidate is timed right after its allocation by
-        two untimed and four timed streaming-only steps (its contents are
-        undefined until the populations are set).  Until the fastest so far
-        reaches PLACEMENT_GOOD_GBS, a further candidate is allocated while
-        that one is held, right after a spacer allocation of the next size in
+        rates (768^3 FP64: from 0.63 to 0.80 of 8 TB/s), and the step over a
+        buffer runs within 1% of the streaming-only step over it.  A probed
+        candidate is timed right after its allocation by two untimed and four
+        timed streaming-only steps (its contents are undefined until the
+        populations are set).  Until the fastest so far reaches
+        PLACEMENT_GOOD_GBS, a further candidate is allocated while that one
+        is held, right after a spacer allocation of the next size in
         PLACEMENT_SPACERS (so that it starts elsewhere; the spacer is freed
         once the candidate exists), and the slower of the two is freed and
         handed back to the driver (torch.cuda.empty_cache, so that the next
         allocation is not served from torch's cache), up to `tries`
         candidates and as long as the device has room for the spacer, one
-        more candidate, the macro array and PLACEMENT_RESERVE_BYTES.  self.placement records
-        the probed rates in GB/s (algorithmic bytes of a step over its
-        duration) and the index kept."""
+        more candidate, the macro array and PLACEMENT_RESERVE_BYTES.
+        self.placement records the probed rates in GB/s (algorithmic bytes
+        of a step over its duration) and the index kept."""
         size = 27 * n * torch.tensor([], dtype=el).element_size()
         if placement not in ("auto", "probe", "first"):
             raise ValueError(f"placement {placement!r}: expected 'auto', 'probe' or 'first'")
