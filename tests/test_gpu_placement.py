@@ -6,7 +6,7 @@ arguments it rejects."""
 import pytest
 import torch
 
-from pytnl_lbm import D3Q27Lattice, lattice
+from pytnl_lbm import D3Q27Lattice, _abi, lattice
 
 pytestmark = pytest.mark.gpu
 
@@ -60,14 +60,15 @@ def test_probe_arguments_are_checked():
 
 def test_probed_slab_with_overlaps_steps_like_the_first_allocation():
     """A slab with ghost planes (the x-distributed share) is probed over its
-    local sites; the distributed step afterwards matches the unprobed one."""
+    local sites; the distributed step afterwards (ghost planes read, here at
+    their equilibrium values) matches the unprobed one."""
     out = []
     for placement in ("first", "probe"):
         lat = D3Q27Lattice(12, 20, 24, nu=0.01, overlap=(1, 0, 0), placement=placement, placement_tries=2)
         assert (lat.placement is None) == (placement == "first")
         lat.set_equilibrium(1.0, 0.02, 0.01, -0.03)
         for _ in range(3):
-            lat.step()
+            lat.step(flags=_abi.FLAG_DISTRIBUTED_X)
         torch.cuda.synchronize()
         out.append(lat.export_f64().cpu())
     assert torch.equal(out[0], out[1])
